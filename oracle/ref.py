@@ -1,0 +1,128 @@
+"""ctypes access to oracle/_ref/libkemoref.so: the reference's own numerical
+routines compiled from their sources (oracle/Makefile.ref) behind this
+project's driver (oracle/ref_harness.f90).  TEST INFRASTRUCTURE ONLY; tests
+that use it skip when the library has not been built (it is built only where
+the reference tree is present)."""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(_HERE, "_ref", "libkemoref.so")
+REFERENCE_TREE = "/root/reference/MHD"
+_LIB = None
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def build() -> bool:
+    """Build _ref when the reference tree is present; returns availability."""
+    if os.path.isdir(REFERENCE_TREE):
+        subprocess.run(["make", "-s", "-f", "Makefile.ref"], cwd=_HERE, check=True)
+    return os.path.exists(SO)
+
+
+def available() -> bool:
+    return os.path.exists(SO)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        L = ctypes.CDLL(SO)
+        L.ref_gauss_points.argtypes = [ctypes.c_int, _dp, _dp]
+        L.ref_schmidt_legendres_m.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp]
+        L.ref_sph_normalizations.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+        L.ref_rfftm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+        L.ref_setup.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+        L.ref_get_table.argtypes = [ctypes.c_int, _dp]
+        L.ref_leg_bwd.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.ref_leg_fwd.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def gauss_points(n):
+    x = np.empty(n)
+    w = np.empty(n)
+    lib().ref_gauss_points(n, _p(x), _p(w))
+    return x, w
+
+
+def schmidt_legendres_m(ltr, m, theta):
+    p = np.zeros(ltr + 1)
+    dp = np.zeros(ltr + 1)
+    lib().ref_schmidt_legendres_m(ltr, m, theta, _p(p), _p(dp))
+    return p, dp
+
+
+def sph_normalizations(l, m):
+    g = np.empty(17)
+    lib().ref_sph_normalizations(l, m, _p(g))
+    return g
+
+
+def rfft_fwd(x):
+    """Lines contiguous in, FFTPACK halfcomplex lines out."""
+    x = np.asarray(x, dtype=np.float64)
+    n = x.shape[-1]
+    lines = x.reshape(-1, n)
+    buf = np.ascontiguousarray(lines.T)  # (n, lot): line index fastest
+    lib().ref_rfftm(1, n, lines.shape[0], _p(buf))
+    return np.ascontiguousarray(buf.T).reshape(x.shape)
+
+
+def rfft_bwd(c):
+    c = np.asarray(c, dtype=np.float64)
+    n = c.shape[-1]
+    lines = c.reshape(-1, n)
+    buf = np.ascontiguousarray(lines.T)
+    lib().ref_rfftm(-1, n, lines.shape[0], _p(buf))
+    return np.ascontiguousarray(buf.T).reshape(c.shape)
+
+
+class RefPlan:
+    """Module-level state in the harness: one live plan at a time."""
+
+    def __init__(self, L, nr, nth, radius):
+        radius = np.ascontiguousarray(radius, dtype=np.float64)
+        self.L, self.nr, self.nth = L, nr, nth
+        self.jmax = (L + 1) ** 2
+        self.nslot = 2 * L + 1
+        self.nth_h = (nth + 1) // 2
+        lib().ref_setup(L, nr, nth, _p(radius))
+
+    def table(self, name):
+        which = {"gpoint": 1, "weight": 2, "P": 3, "dP": 4, "Ps": 5, "dPs": 6, "g": 7}[name]
+        size = {1: self.nth, 2: self.nth, 3: self.nth * self.jmax, 4: self.nth * self.jmax,
+                5: self.nth_h * self.jmax, 6: self.nth_h * self.jmax, 7: 17 * self.jmax}[which]
+        out = np.empty(size)
+        lib().ref_get_table(which, _p(out))
+        if which in (3, 4):
+            out = out.reshape(self.jmax, self.nth)
+        elif which in (5, 6):
+            out = out.reshape(self.jmax, self.nth_h)
+        elif which == 7:
+            out = out.reshape(17, self.jmax)
+        return out
+
+    def leg_bwd(self, nv, ns, sp):
+        ncomp = 3 * nv + ns
+        WR = np.ascontiguousarray(sp, dtype=np.float64).ravel().copy()
+        WS = np.zeros(ncomp * self.nth * self.nr * self.nslot)
+        lib().ref_leg_bwd(nv, ns, _p(WR), _p(WS))
+        return WS
+
+    def leg_fwd(self, nv, ns, vr):
+        ncomp = 3 * nv + ns
+        WR = np.ascontiguousarray(vr, dtype=np.float64).ravel().copy()
+        WS = np.zeros(ncomp * self.jmax * self.nr)
+        lib().ref_leg_fwd(nv, ns, _p(WR), _p(WS))
+        return WS
