@@ -1,0 +1,130 @@
+/*
+ * kemorin_sht.h -- C ABI of the MI355X (gfx950) spherical-harmonic transform
+ * engine that replaces the hot path of Kemorin/Calypso's sph_trans.
+ *
+ * Plain C: opaque plan handle, plain pointers and sizes, int status codes.
+ * Reference files are cited relative to the reference tree, with
+ *   SS/ = MHD/Fortran_src/PARALLEL_src/SPH_SHELL_src/
+ *   SR/ = MHD/Fortran_src/SOLVER_src/send_recv/
+ *
+ * Data layouts (single rank; identical to the reference's buffers):
+ *   spectral buffer (rj send/recv, rlm):   sp[nd + ncomp*(j + jmax*k)]
+ *       j: single-rank rlm order (m from -L to L, then l = |m|..L),
+ *       k: radius, nd: component (vectors first as (pol, dpol/dr, tor)
+ *       triples, then scalars).  jmax = (L+1)^2.
+ *   rtm buffer (Legendre output/input):    v[nd + ncomp*(t + nth*(k + nr*s))]
+ *       t: Gauss colatitude (ascending x = cos(theta)), s = m + L.
+ *   rtp spectral buffer (FFT input/output): v[nd + ncomp*(k + nr*(t + nth*p))]
+ *       p: FFTPACK slot, a0 -> 0, Nyquist -> 1, cos(m phi) -> 2m,
+ *       sin(m phi) -> 2m+1.
+ *   grid field (v_rtp):                    v[(k + nr*(t + nth*i)) + nnod*nd]
+ *       i: longitude phi_i = 2 pi i / nphi, nnod = nr*nth*nphi.
+ * All pointers passed to the transform entry points are DEVICE pointers
+ * (hipMalloc'd or torch CUDA tensors) unless the name ends in _host.
+ * Calls on one plan are stream-ordered on the plan's stream and not
+ * re-entrant; different plans may run concurrently.
+ */
+#ifndef KEMORIN_SHT_H
+#define KEMORIN_SHT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SHT_OK = 0,
+  SHT_ERR_ARG = 1,         /* bad descriptor, size or null pointer   */
+  SHT_ERR_UNSUPPORTED = 2, /* e.g. FFT length with a prime factor > 16 */
+  SHT_ERR_OOM = 3,         /* device allocation failed               */
+  SHT_ERR_HIP = 4,         /* HIP runtime error                      */
+  SHT_ERR_STATE = 5        /* plan not ready / wrong precision mode  */
+};
+
+enum { SHT_PREC_FP64 = 0, SHT_PREC_SPLIT_INT8 = 1 };
+
+typedef struct sht_plan sht_plan;
+
+typedef struct {
+  int l_truncation; /* L: truncation degree (triangular)                */
+  int nidx_r;       /* Nr: number of radial points                       */
+  int nidx_theta;   /* Ntheta: Gauss-Legendre colatitudes                */
+  int nidx_phi;     /* Nphi: longitudes (even, > 2L+1)                   */
+  const double *radius; /* host array of Nr radii                        */
+  int max_ncomp;    /* largest ncomp a call will pass (workspace size)    */
+  int precision;    /* SHT_PREC_FP64 or SHT_PREC_SPLIT_INT8              */
+  int n_slices;     /* split mode: INT8 slices per operand (0 = default) */
+  int flags;        /* reserved, pass 0                                  */
+} sht_desc;
+
+/* Replaces initialize_sph_trans (SS/init_sph_trans.f90:46-74) and the
+ * Legendre/FFT set-up it runs: Gauss points with the reference's Newton
+ * loop (MHD/Fortran_src/CORE_src/gauss_integration/gauss_integration.f90:60),
+ * Schmidt P_l^m and dP/dtheta generated on the GPU in FP64, 1/sin(theta),
+ * index stacks and FFT twiddles.  The plan owns all device memory. */
+int sht_plan_create(const sht_desc *desc, sht_plan **plan);
+int sht_plan_destroy(sht_plan *plan);
+/* Run subsequent work on this hipStream_t (NULL = plan's own stream). */
+int sht_plan_set_stream(sht_plan *plan, void *hip_stream);
+/* Copy a plan table to host: "gpoint","weight","colat","asin_t","radius"
+ * (double) or "lstack" (int, 2L+2 entries).  n = capacity in elements. */
+int sht_plan_get_table(const sht_plan *plan, const char *name, void *out, long long n);
+/* Device bytes held by the plan (tables + workspace). */
+long long sht_plan_device_bytes(const sht_plan *plan);
+
+/* Whole transforms on device-resident buffers.
+ * Replaces sph_backward_transforms (SS/t_sph_transforms.f90:74-123):
+ * d_WS_rj is the rj->rlm send buffer (spectral layout above, ncomp =
+ * 3*nvector + nscalar), d_v_rtp receives the grid fields. */
+int sht_backward(sht_plan *plan, int nvector, int nscalar,
+                 const double *d_WS_rj, double *d_v_rtp);
+/* Replaces sph_forward_transforms (SS/t_sph_transforms.f90:250-297):
+ * d_v_rtp grid fields in, d_WR_rj (rlm->rj receive buffer) out. */
+int sht_forward(sht_plan *plan, int nvector, int nscalar,
+                const double *d_v_rtp, double *d_WR_rj);
+/* Same, with host buffers (one H2D and one D2H copy; PCIe-bound). */
+int sht_backward_host(sht_plan *plan, int nvector, int nscalar,
+                      const double *WS_rj, double *v_rtp);
+int sht_forward_host(sht_plan *plan, int nvector, int nscalar,
+                     const double *v_rtp, double *WR_rj);
+
+/* Stage entry points on the reference's buffers, for callers that run work
+ * between stages (the Coriolis hooks of MS/sph_trans_w_coriols.f90:146,233).
+ * Legendre: replaces sel_backward_legendre_trans / sel_forward_legendre_trans
+ * (SS/t_legendre_trans_select.f90:184,273) in symmetric_BLAS_big mode. */
+int sht_legendre_backward(sht_plan *plan, int nvector, int nscalar,
+                          const double *d_WR_rlm, double *d_WS_rtm);
+int sht_legendre_forward(sht_plan *plan, int nvector, int nscalar,
+                         const double *d_WR_rtm, double *d_WS_rlm);
+/* FFT: replaces back_FFT_select_from_recv / fwd_FFT_select_to_send
+ * (SS/t_sph_FFT_selector.F90:258,214) for FFTPACK (SS/t_sph_FFTPACK5.f90:
+ * 183-331).  The forward writes slots of orders m <= L and zeroes the rest. */
+int sht_fft_backward(sht_plan *plan, int ncomp, const double *d_WR_rtp,
+                     double *d_v_rtp);
+int sht_fft_forward(sht_plan *plan, int ncomp, const double *d_v_rtp,
+                    double *d_WS_rtp);
+/* Exchanges: single-rank calypso_sph_comm_N (SS/spherical_SRs_N.f90:233)
+ * followed by the receive-side gather; bit-exact permutations.
+ * rtm -> rtp fills slots with no Legendre order with zero (the dummy slot
+ * of SR/set_from_recv_buf_rev.f90:77-87). */
+int sht_exchange_rtm_to_rtp(sht_plan *plan, int ncomp, const double *d_rtm,
+                            double *d_rtp);
+int sht_exchange_rtp_to_rtm(sht_plan *plan, int ncomp, const double *d_rtp,
+                            double *d_rtm);
+/* Generic receive gather (SR/set_from_recv_buf_rev.f90:77-87):
+ * dst[nd + ncomp*i] = irev[i] == n_src+1 ? 0 : src[nd + ncomp*(irev[i]-1)],
+ * i < n_dst, irev 1-based (device int array). */
+int sht_exchange_gather_rev(sht_plan *plan, int ncomp, long long n_dst,
+                            long long n_src, const int *d_irev,
+                            const double *d_src, double *d_dst);
+
+/* Diagnostics: checks the MFMA lane layouts this library assumes on the
+ * current device (FP64 16x16x4 and INT8 16x16x64).  0 = as assumed. */
+int sht_selftest(void);
+/* Last error message of the calling thread. */
+int sht_get_last_error(char *buf, int n);
+const char *sht_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

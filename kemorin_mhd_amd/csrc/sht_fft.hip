@@ -1,0 +1,333 @@
+// sht_fft.hip -- zonal real FFT along phi in the FFTPACK5.1 convention
+// (MHD/External_libs/FFTPACK5.1D/fftpack5.1d.f90:9288-9335; slot use of
+// SS/t_sph_FFTPACK5.f90:183-331):
+//   forward  a0 = (1/N) sum x_j,  a_m = (2/N) sum x_j cos(2 pi j m/N),
+//            b_m = (2/N) sum x_j sin(2 pi j m/N)
+//   backward x_j = a0 + sum_{m>=1} (a_m cos + b_m sin)   (m <= L only).
+// Two real lines are packed as the real and imaginary parts of one complex
+// line; a work-group runs several complex Stockham FFTs (mixed radix 8/4/2/3/
+// 5 and a direct DFT for other primes up to 13) in LDS.  The Fourier side
+// holds orders m <= L in the compact order q = 0 (a0), 2m-1 (a_m), 2m (b_m);
+// the grid side is the reference's field-major v_rtp with phi slowest.
+#include <cmath>
+
+#include "sht_internal.h"
+
+namespace ksht {
+
+constexpr int kFftThreads = 256;
+constexpr int kFftElems = 6144;  // complex elements per work-group in LDS
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// multiply by s*i (s = +1 or -1)
+__device__ __forceinline__ double2 cmuli(double2 a, double s) { return make_double2(-s * a.y, s * a.x); }
+
+template <int R>
+__device__ __forceinline__ void dft_small(double2 *v, double s) {
+  if constexpr (R == 2) {
+    const double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    const double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    const double2 t2 = cadd(v[1], v[3]), t3 = cmuli(csub(v[1], v[3]), s);
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  } else if constexpr (R == 8) {
+    double2 a[4] = {v[0], v[2], v[4], v[6]}, b[4] = {v[1], v[3], v[5], v[7]};
+    dft_small<4>(a, s);
+    dft_small<4>(b, s);
+    const double h = 0.70710678118654752440;
+    // w^k b_k with w = exp(s*2*pi*i/8)
+    const double2 b1 = make_double2(h * (b[1].x - s * b[1].y), h * (b[1].y + s * b[1].x));
+    const double2 b2 = cmuli(b[2], s);
+    const double2 b3 = make_double2(h * (-b[3].x - s * b[3].y), h * (-b[3].y + s * b[3].x));
+    v[0] = cadd(a[0], b[0]);
+    v[4] = csub(a[0], b[0]);
+    v[1] = cadd(a[1], b1);
+    v[5] = csub(a[1], b1);
+    v[2] = cadd(a[2], b2);
+    v[6] = csub(a[2], b2);
+    v[3] = cadd(a[3], b3);
+    v[7] = csub(a[3], b3);
+  } else if constexpr (R == 3) {
+    const double c = -0.5, sn = 0.86602540378443864676 * s;
+    const double2 t = cadd(v[1], v[2]), u = csub(v[1], v[2]);
+    const double2 m = make_double2(v[0].x + c * t.x, v[0].y + c * t.y);
+    const double2 iu = make_double2(-sn * u.y, sn * u.x);
+    v[0] = cadd(v[0], t);
+    v[1] = cadd(m, iu);
+    v[2] = csub(m, iu);
+  } else {
+    double2 o[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      double2 acc = v[0];
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        const int e = (r * q) % R;
+        double sv, cv;
+        sincospi(2.0 * e / R, &sv, &cv);
+        acc = cadd(acc, cmul(v[r], make_double2(cv, s * sv)));
+      }
+      o[q] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < R; q++) v[q] = o[q];
+  }
+}
+
+// One Stockham pass over F complex FFTs of length n held in LDS.
+template <int R>
+__device__ void fft_pass(double2 *buf, int n, int F, int Ns, const double2 *__restrict__ tw,
+                         double s) {
+  constexpr int NBT = (kFftElems / kFftThreads + R - 1) / R;
+  const int nR = n / R, nb = F * nR;
+  double2 v[NBT][R];
+#pragma unroll
+  for (int it = 0; it < NBT; it++) {
+    const int bi = threadIdx.x + it * kFftThreads;
+    if (bi < nb) {
+      const int f = bi / nR, j = bi - f * nR;
+      const int k = j % Ns;
+      const double2 *src = buf + (long)f * n + j;
+#pragma unroll
+      for (int r = 0; r < R; r++) v[it][r] = src[r * nR];
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        double2 w = tw[k * R + r];
+        w.y *= -s;  // table holds exp(-2 pi i k r / (Ns R))
+        v[it][r] = cmul(v[it][r], w);
+      }
+      dft_small<R>(v[it], s);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < NBT; it++) {
+    const int bi = threadIdx.x + it * kFftThreads;
+    if (bi < nb) {
+      const int f = bi / nR, j = bi - f * nR;
+      const int k = j % Ns;
+      double2 *dst = buf + (long)f * n + (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) dst[r * Ns] = v[it][r];
+    }
+  }
+  __syncthreads();
+}
+
+__device__ void fft_run(double2 *buf, const FftPlan &fp, int F, const double2 *__restrict__ tw,
+                        double s) {
+  for (int ip = 0; ip < fp.npass; ip++) {
+    const FftPass ps = fp.pass[ip];
+    const double2 *t = tw + ps.tw_off;
+    switch (ps.R) {
+      case 8: fft_pass<8>(buf, fp.n, F, ps.Ns, t, s); break;
+      case 4: fft_pass<4>(buf, fp.n, F, ps.Ns, t, s); break;
+      case 2: fft_pass<2>(buf, fp.n, F, ps.Ns, t, s); break;
+      case 3: fft_pass<3>(buf, fp.n, F, ps.Ns, t, s); break;
+      case 5: fft_pass<5>(buf, fp.n, F, ps.Ns, t, s); break;
+      case 7: fft_pass<7>(buf, fp.n, F, ps.Ns, t, s); break;
+      case 11: fft_pass<11>(buf, fp.n, F, ps.Ns, t, s); break;
+      case 13: fft_pass<13>(buf, fp.n, F, ps.Ns, t, s); break;
+      default: break;
+    }
+  }
+}
+
+// Fourier coefficients -> grid.  blockIdx.x: block of B lines, blockIdx.y: nd.
+__global__ void __launch_bounds__(kFftThreads) k_fft_bwd(
+    FftPlan fp, int L, long long nline, const double *__restrict__ X, FourierAddr xa,
+    double *__restrict__ v, const double2 *__restrict__ tw) {
+  extern __shared__ double2 buf[];
+  const int n = fp.n, B = fp.lines_per_wg, F = B / 2;
+  const int nd = blockIdx.y;
+  const long long line0 = (long long)blockIdx.x * B;
+  const long long nnod = nline * n;
+  // zero the orders above L
+  for (int idx = threadIdx.x; idx < F * (n - 2 * L - 1); idx += kFftThreads) {
+    const int f = idx / (n - 2 * L - 1), k = L + 1 + idx % (n - 2 * L - 1);
+    buf[f * n + k] = make_double2(0.0, 0.0);
+  }
+  for (int idx = threadIdx.x; idx < F * (L + 1); idx += kFftThreads) {
+    const int m = idx / F, f = idx - m * F;
+    const long long la = line0 + 2 * f, lb = la + 1;
+    const long long base = (long long)nd * xa.s_nd;
+    double aa = 0, ba = 0, ab = 0, bb = 0;
+    if (m == 0) {
+      const long long o = xa.q_off[0] * xa.q_mul + base;
+      if (la < nline) aa = X[o + la * xa.s_line];
+      if (lb < nline) ab = X[o + lb * xa.s_line];
+      buf[f * n] = make_double2(aa, ab);
+    } else {
+      const long long oc = xa.q_off[2 * m - 1] * xa.q_mul + base;
+      const long long os = xa.q_off[2 * m] * xa.q_mul + base;
+      if (la < nline) {
+        aa = X[oc + la * xa.s_line];
+        ba = X[os + la * xa.s_line];
+      }
+      if (lb < nline) {
+        ab = X[oc + lb * xa.s_line];
+        bb = X[os + lb * xa.s_line];
+      }
+      // C = (a - i b)/2 per line; Z_m = C_a + i C_b, Z_{n-m} = conj(C_a) + i conj(C_b)
+      buf[f * n + m] = make_double2(0.5 * (aa + bb), 0.5 * (ab - ba));
+      buf[f * n + n - m] = make_double2(0.5 * (aa - bb), 0.5 * (ab + ba));
+    }
+  }
+  __syncthreads();
+  fft_run(buf, fp, F, tw, +1.0);
+  for (int idx = threadIdx.x; idx < F * n; idx += kFftThreads) {
+    const int j = idx / F, f = idx - j * F;
+    const long long la = line0 + 2 * f;
+    const double2 z = buf[f * n + j];
+    double *o = v + (long long)nd * nnod + (long long)j * nline + la;
+    if (la + 1 < nline) {
+      if ((nline & 1) == 0)
+        *reinterpret_cast<double2 *>(o) = z;  // 16-byte aligned: la and nline even
+      else {
+        o[0] = z.x;
+        o[1] = z.y;
+      }
+    } else if (la < nline) {
+      o[0] = z.x;
+    }
+  }
+}
+
+// grid -> Fourier coefficients (orders m <= L).
+__global__ void __launch_bounds__(kFftThreads) k_fft_fwd(
+    FftPlan fp, int L, long long nline, const double *__restrict__ v, double *__restrict__ X,
+    FourierAddr xa, const double2 *__restrict__ tw) {
+  extern __shared__ double2 buf[];
+  const int n = fp.n, B = fp.lines_per_wg, F = B / 2;
+  const int nd = blockIdx.y;
+  const long long line0 = (long long)blockIdx.x * B;
+  const long long nnod = nline * n;
+  for (int idx = threadIdx.x; idx < F * n; idx += kFftThreads) {
+    const int j = idx / F, f = idx - j * F;
+    const long long la = line0 + 2 * f;
+    const double *src = v + (long long)nd * nnod + (long long)j * nline + la;
+    double2 z = make_double2(0.0, 0.0);
+    if (la + 1 < nline) {
+      if ((nline & 1) == 0)
+        z = *reinterpret_cast<const double2 *>(src);
+      else
+        z = make_double2(src[0], src[1]);
+    } else if (la < nline) {
+      z.x = src[0];
+    }
+    buf[f * n + j] = z;
+  }
+  __syncthreads();
+  fft_run(buf, fp, F, tw, -1.0);
+  const double sn = 1.0 / (double)n, tsn = 2.0 / (double)n;
+  for (int idx = threadIdx.x; idx < F * (L + 1); idx += kFftThreads) {
+    const int m = idx / F, f = idx - m * F;
+    const long long la = line0 + 2 * f, lb = la + 1;
+    const double2 zm = buf[f * n + m];
+    const double2 zn = buf[f * n + (m == 0 ? 0 : n - m)];
+    // X_a = (Z_m + conj Z_{n-m})/2, X_b = -i (Z_m - conj Z_{n-m})/2
+    const double xar = 0.5 * (zm.x + zn.x), xai = 0.5 * (zm.y - zn.y);
+    const double xbr = 0.5 * (zm.y + zn.y), xbi = -0.5 * (zm.x - zn.x);
+    const long long base = (long long)nd * xa.s_nd;
+    if (m == 0) {
+      const long long o = xa.q_off[0] * xa.q_mul + base;
+      if (la < nline) X[o + la * xa.s_line] = sn * xar;
+      if (lb < nline) X[o + lb * xa.s_line] = sn * xbr;
+    } else {
+      const long long oc = xa.q_off[2 * m - 1] * xa.q_mul + base;
+      const long long os = xa.q_off[2 * m] * xa.q_mul + base;
+      if (la < nline) {
+        X[oc + la * xa.s_line] = tsn * xar;
+        X[os + la * xa.s_line] = -tsn * xai;
+      }
+      if (lb < nline) {
+        X[oc + lb * xa.s_line] = tsn * xbr;
+        X[os + lb * xa.s_line] = -tsn * xbi;
+      }
+    }
+  }
+}
+
+int fft_plan_init(sht_plan *p) {
+  FftPlan &fp = p->fft;
+  const int n = p->nphi;
+  fp.n = n;
+  fp.npass = 0;
+  int rem = n;
+  std::vector<int> fac;
+  while (rem % 8 == 0) { fac.push_back(8); rem /= 8; }
+  while (rem % 4 == 0) { fac.push_back(4); rem /= 4; }
+  while (rem % 2 == 0) { fac.push_back(2); rem /= 2; }
+  for (int q : {3, 5, 7, 11, 13})
+    while (rem % q == 0) { fac.push_back(q); rem /= q; }
+  if (rem != 1 || (int)fac.size() > kMaxPass) {
+    set_error("FFT length %d has a prime factor > 13 (unsupported)", n);
+    return SHT_ERR_UNSUPPORTED;
+  }
+  int F = kFftElems / n;  // complex FFTs per work-group
+  if (F < 1) {
+    set_error("FFT length %d exceeds the LDS capacity (%d)", n, kFftElems);
+    return SHT_ERR_UNSUPPORTED;
+  }
+  if (F > 32) F = 32;
+  int f2 = 1;
+  while (f2 * 2 <= F) f2 *= 2;
+  fp.lines_per_wg = 2 * f2;
+  std::vector<double2> tw;
+  int Ns = 1;
+  for (int R : fac) {
+    FftPass ps;
+    ps.R = R;
+    ps.Ns = Ns;
+    ps.tw_off = (int)tw.size();
+    ps.pad = 0;
+    for (int k = 0; k < Ns; k++)
+      for (int r = 0; r < R; r++) {
+        const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)(k * r) /
+                              (long double)(Ns * R);
+        tw.push_back(make_double2((double)cosl(a), (double)sinl(a)));
+      }
+    fp.pass[fp.npass++] = ps;
+    Ns *= R;
+  }
+  const int lds = (int)(sizeof(double2) * (size_t)f2 * n);
+  KSHT_HIP(hipFuncSetAttribute((const void *)k_fft_bwd, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  KSHT_HIP(hipFuncSetAttribute((const void *)k_fft_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  KSHT_HIP(hipMalloc(&p->t.tw, sizeof(double2) * tw.size()));
+  KSHT_HIP(hipMemcpy(p->t.tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice));
+  p->device_bytes += (long long)(sizeof(double2) * tw.size());
+  return SHT_OK;
+}
+
+int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, double *v) {
+  const long long nline = (long long)p->nr * p->nth;
+  const int B = p->fft.lines_per_wg;
+  const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);
+  const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.n;
+  hipLaunchKernelGGL(k_fft_bwd, grid, dim3(kFftThreads), lds, p->stream, p->fft, p->L, nline, X,
+                     xa, v, p->t.tw);
+  KSHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+
+int launch_fft_fwd(sht_plan *p, int ncomp, const double *v, double *X, FourierAddr xa) {
+  const long long nline = (long long)p->nr * p->nth;
+  const int B = p->fft.lines_per_wg;
+  const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);
+  const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.n;
+  hipLaunchKernelGGL(k_fft_fwd, grid, dim3(kFftThreads), lds, p->stream, p->fft, p->L, nline, v,
+                     X, xa, p->t.tw);
+  KSHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+
+}  // namespace ksht

@@ -1,0 +1,127 @@
+// sht_internal.h -- plan state and kernel launch interfaces shared by the
+// translation units of libkemorin_sht.so (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/kemorin_sht.h"
+
+namespace ksht {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef int i4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTile = 16;   // MFMA tile edge (rows, radii, degrees)
+constexpr int kMaxPass = 24;
+
+void set_error(const char *fmt, ...);
+
+#define KSHT_HIP(call)                                                   \
+  do {                                                                   \
+    hipError_t e_ = (call);                                              \
+    if (e_ != hipSuccess) {                                              \
+      ::ksht::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,       \
+                        hipGetErrorString(e_));                          \
+      return SHT_ERR_HIP;                                                \
+    }                                                                    \
+  } while (0)
+
+// One pass of the Stockham FFT: radix, span of finished sub-transforms and
+// offset of its twiddle block (Ns*R entries) in the twiddle table.
+struct FftPass {
+  int R, Ns, tw_off, pad;
+};
+struct FftPlan {
+  int n, npass, lines_per_wg, pad;
+  FftPass pass[kMaxPass];
+};
+
+// Addressing of an (order m, colatitude t, radius k, component nd) element
+// of a Legendre-side buffer:
+//   slot_off[m + L]*slot_mul + nd*s_nd + t*s_t + k*s_k.
+struct GridAddr {
+  const long long *slot_off;
+  long long slot_mul, s_nd, s_t, s_k;
+};
+// Addressing of a spectral element (rlm index j, radius k, component nd):
+// nd*s_nd + j*s_j + k*s_k.
+struct SpecAddr {
+  long long s_nd, s_j, s_k;
+};
+// Addressing of an FFT-side element: q_off[q]*q_mul + nd*s_nd + line*s_line,
+// q = 0 (a0), 2m-1 (cos m), 2m (sin m), line = k + nr*t.
+struct FourierAddr {
+  const long long *q_off;
+  long long q_mul, s_nd, s_line;
+};
+// Grid field (v_rtp): line + nline*i + nnod*nd.
+
+struct Tables {
+  // per order M = 0..L: offsets (in doubles) of the even/odd blocks of the
+  // backward (i-major, theta fastest) and forward (theta-major, i fastest)
+  // P tables, and the padded degree counts.
+  long long *offb = nullptr, *offf = nullptr;  // device, 2*(L+1) each
+  int *ipad = nullptr;                          // device, 2*(L+1)
+  int *jst = nullptr;                           // device, 2L+1: lstack[s]
+  double *Pb = nullptr, *dPb = nullptr, *Pf = nullptr, *dPf = nullptr;
+  double *radius = nullptr, *a_r = nullptr;     // device, Nr
+  double *weight = nullptr, *asin_t = nullptr;  // device, Ntheta
+  double *cos_t = nullptr, *sin_t = nullptr;    // device, Ntheta
+  double2 *tw = nullptr;                        // FFT twiddles
+  long long *slot_off_X = nullptr, *slot_off_rtm = nullptr;  // 2L+1 each
+  long long *q_off_X = nullptr, *q_off_rtp = nullptr;        // 2L+1 each
+};
+
+}  // namespace ksht
+
+struct sht_plan {
+  int L, nr, nth, nphi, nslot, jmax, nth_h, nth_hp, nr_p, max_ncomp;
+  int precision, n_slices;
+  int device;
+  hipStream_t stream;
+  hipStream_t own_stream;
+  std::vector<double> radius, gpoint, weight, colat, asin_t;
+  std::vector<int> lstack;
+  std::vector<long long> h_offb, h_offf;
+  std::vector<int> h_ipad;
+  long long p_table_elems_b, p_table_elems_f;
+  ksht::Tables t;
+  ksht::FftPlan fft;
+  double *X = nullptr;        // field-major Fourier workspace
+  long long x_elems = 0;
+  long long device_bytes = 0;
+  // split-precision state (allocated only in SHT_PREC_SPLIT_INT8 mode)
+  void *split = nullptr;
+};
+
+namespace ksht {
+// Legendre launchers (sht_legendre.hip)
+int launch_leg_bwd(sht_plan *p, int nv, int ns, const double *sp, SpecAddr sa,
+                   double *out, GridAddr ga);
+int launch_leg_fwd(sht_plan *p, int nv, int ns, const double *in, GridAddr ga,
+                   double *sp, SpecAddr sa);
+// FFT launchers (sht_fft.hip)
+int fft_plan_init(sht_plan *p);
+int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa,
+                   double *v);
+int launch_fft_fwd(sht_plan *p, int ncomp, const double *v, double *X,
+                   FourierAddr xa);
+// reorders (sht_reorder.hip)
+int launch_rtm_to_rtp(sht_plan *p, int ncomp, const double *rtm, double *rtp);
+int launch_rtp_to_rtm(sht_plan *p, int ncomp, const double *rtp, double *rtm);
+int launch_gather_rev(sht_plan *p, int ncomp, long long n_dst, long long n_src,
+                      const int *irev, const double *src, double *dst);
+// P table generation (sht_plan.hip)
+int generate_p_tables(sht_plan *p);
+// split precision (sht_split.hip)
+int split_init(sht_plan *p);
+void split_free(sht_plan *p);
+int launch_leg_bwd_split(sht_plan *p, int nv, int ns, const double *sp,
+                         SpecAddr sa, double *out, GridAddr ga);
+int launch_leg_fwd_split(sht_plan *p, int nv, int ns, const double *in,
+                         GridAddr ga, double *sp, SpecAddr sa);
+}  // namespace ksht

@@ -1,0 +1,299 @@
+"""Python host mirror of the reference's sph_trans call surface.
+
+The reference Fortran calls, per time step, ``sph_backward_transforms`` and
+``sph_forward_transforms`` (SS/t_sph_transforms.f90:74,250) after
+``initialize_sph_trans`` (SS/init_sph_trans.f90:46).  :class:`SphTransPlan`
+offers the same operations on device-resident torch tensors through the C ABI
+of ``libkemorin_sht.so`` (``include/kemorin_sht.h``); tensors are used only to
+carry device memory and the stream, all arithmetic runs in the HIP kernels.
+There is no CPU fallback: if the library or a GPU is missing the calls raise.
+
+SS/ = MHD/Fortran_src/PARALLEL_src/SPH_SHELL_src/ in the reference tree.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkemorin_sht.so")
+
+SHT_OK = 0
+SHT_PREC_FP64 = 0
+SHT_PREC_SPLIT_INT8 = 1
+
+_LIB = None
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class ShtError(RuntimeError):
+    """Non-zero status from the C ABI (message from sht_get_last_error)."""
+
+    def __init__(self, status: int, where: str):
+        msg = last_error()
+        super().__init__(f"{where}: status {status}: {msg}")
+        self.status = status
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [("l_truncation", ctypes.c_int), ("nidx_r", ctypes.c_int),
+                ("nidx_theta", ctypes.c_int), ("nidx_phi", ctypes.c_int),
+                ("radius", _dp), ("max_ncomp", ctypes.c_int), ("precision", ctypes.c_int),
+                ("n_slices", ctypes.c_int), ("flags", ctypes.c_int)]
+
+
+def lib() -> ctypes.CDLL:
+    """Load the in-tree HIP library (fails loudly when it is not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run python -m kemorin_mhd_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        L.sht_plan_create.argtypes = [ctypes.POINTER(_Desc), ctypes.POINTER(_vp)]
+        L.sht_plan_destroy.argtypes = [_vp]
+        L.sht_plan_set_stream.argtypes = [_vp, _vp]
+        L.sht_plan_get_table.argtypes = [_vp, ctypes.c_char_p, _vp, ctypes.c_longlong]
+        L.sht_plan_device_bytes.argtypes = [_vp]
+        L.sht_plan_device_bytes.restype = ctypes.c_longlong
+        for nm in ("sht_backward", "sht_forward", "sht_backward_host", "sht_forward_host",
+                   "sht_legendre_backward", "sht_legendre_forward"):
+            getattr(L, nm).argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]
+        for nm in ("sht_fft_backward", "sht_fft_forward", "sht_exchange_rtm_to_rtp",
+                   "sht_exchange_rtp_to_rtm"):
+            getattr(L, nm).argtypes = [_vp, ctypes.c_int, _vp, _vp]
+        L.sht_exchange_gather_rev.argtypes = [_vp, ctypes.c_int, ctypes.c_longlong,
+                                              ctypes.c_longlong, _vp, _vp, _vp]
+        L.sht_get_last_error.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        L.sht_version.restype = ctypes.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(512)
+    lib().sht_get_last_error(buf, 512)
+    return buf.value.decode(errors="replace")
+
+
+def _check(rc: int, where: str):
+    if rc != SHT_OK:
+        raise ShtError(rc, where)
+
+
+def _ptr(t) -> int:
+    """Device (or host numpy) address of a contiguous float64 buffer."""
+    if isinstance(t, np.ndarray):
+        assert t.dtype == np.float64 and t.flags["C_CONTIGUOUS"]
+        return t.ctypes.data
+    assert t.is_contiguous(), "buffers must be contiguous"
+    return t.data_ptr()
+
+
+def chebyshev_radius(nr: int, ri: float, ro: float) -> np.ndarray:
+    """Chebyshev-Gauss-Lobatto radii of the reference's shell grid
+    (MHD/Fortran_src/CORE_src/SPH_SPECTR_src/chebyshev_radial_grid.f90,
+    set_chebyshev_distance_shell)."""
+    k = np.arange(nr, dtype=np.float64)
+    return ri + 0.5 * (ro - ri) * (1.0 - np.cos(np.pi * k / (nr - 1)))
+
+
+@dataclass
+class ShellConfig:
+    """Resolution of one shell: truncation L, radii, Gauss and phi points."""
+    L: int
+    nr: int
+    nth: int
+    nphi: int
+    ri: float = 0.35 / 0.65
+    ro: float = 1.0 / 0.65
+
+    @property
+    def jmax(self):
+        return (self.L + 1) ** 2
+
+    @property
+    def nslot(self):
+        return 2 * self.L + 1
+
+    def radius(self):
+        return chebyshev_radius(self.nr, self.ri, self.ro)
+
+    def n_rj(self, ncomp):
+        return ncomp * self.jmax * self.nr
+
+    def n_rtp(self, ncomp):
+        return ncomp * self.nr * self.nth * self.nphi
+
+    def n_rtm(self, ncomp):
+        return ncomp * self.nth * self.nr * self.nslot
+
+
+class SphTransPlan:
+    """One shell's transform plan on the current HIP device.
+
+    Mirrors initialize_sph_trans (SS/init_sph_trans.f90:46-74); the transform
+    methods mirror sph_backward_transforms / sph_forward_transforms
+    (SS/t_sph_transforms.f90:74-123, 250-297) and the stage selectors of
+    SS/t_legendre_trans_select.f90 and SS/t_sph_FFT_selector.F90."""
+
+    def __init__(self, cfg: ShellConfig, max_ncomp: int = 9, precision: int = SHT_PREC_FP64,
+                 n_slices: int = 0, radius=None):
+        self.cfg = cfg
+        r = np.ascontiguousarray(cfg.radius() if radius is None else radius, dtype=np.float64)
+        self._radius = r
+        d = _Desc(cfg.L, cfg.nr, cfg.nth, cfg.nphi, r.ctypes.data_as(_dp), max_ncomp, precision,
+                  n_slices, 0)
+        h = _vp()
+        _check(lib().sht_plan_create(ctypes.byref(d), ctypes.byref(h)), "sht_plan_create")
+        self._h = h
+        self.max_ncomp = max_ncomp
+        self.precision = precision
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().sht_plan_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, stream_ptr: int | None):
+        _check(lib().sht_plan_set_stream(self._h, stream_ptr), "sht_plan_set_stream")
+
+    def device_bytes(self) -> int:
+        return int(lib().sht_plan_device_bytes(self._h))
+
+    def table(self, name: str) -> np.ndarray:
+        n = {"gpoint": self.cfg.nth, "weight": self.cfg.nth, "colat": self.cfg.nth,
+             "asin_t": self.cfg.nth, "radius": self.cfg.nr}.get(name)
+        if name == "lstack":
+            out = np.zeros(self.cfg.nslot + 1, dtype=np.int32)
+            _check(lib().sht_plan_get_table(self._h, b"lstack", out.ctypes.data, out.size),
+                   "sht_plan_get_table")
+            return out
+        out = np.zeros(n)
+        _check(lib().sht_plan_get_table(self._h, name.encode(), out.ctypes.data, n),
+               "sht_plan_get_table")
+        return out
+
+    # whole transforms -------------------------------------------------------
+    def sph_backward_transforms(self, nvector: int, nscalar: int, WS, v_rtp):
+        """spectra (rj send buffer) -> grid fields, device buffers."""
+        _check(lib().sht_backward(self._h, nvector, nscalar, _ptr(WS), _ptr(v_rtp)),
+               "sht_backward")
+
+    def sph_forward_transforms(self, nvector: int, nscalar: int, v_rtp, WR):
+        """grid fields -> spectra (rj receive buffer), device buffers."""
+        _check(lib().sht_forward(self._h, nvector, nscalar, _ptr(v_rtp), _ptr(WR)),
+               "sht_forward")
+
+    def backward_host(self, nvector, nscalar, WS: np.ndarray) -> np.ndarray:
+        ncomp = 3 * nvector + nscalar
+        out = np.empty(self.cfg.n_rtp(ncomp))
+        WS = np.ascontiguousarray(WS, dtype=np.float64).ravel()
+        _check(lib().sht_backward_host(self._h, nvector, nscalar, WS.ctypes.data, out.ctypes.data),
+               "sht_backward_host")
+        return out
+
+    def forward_host(self, nvector, nscalar, v: np.ndarray) -> np.ndarray:
+        ncomp = 3 * nvector + nscalar
+        out = np.empty(self.cfg.n_rj(ncomp))
+        v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+        _check(lib().sht_forward_host(self._h, nvector, nscalar, v.ctypes.data, out.ctypes.data),
+               "sht_forward_host")
+        return out
+
+    # stages -------------------------------------------------------------------
+    def legendre_backward(self, nvector, nscalar, WR_rlm, WS_rtm):
+        _check(lib().sht_legendre_backward(self._h, nvector, nscalar, _ptr(WR_rlm), _ptr(WS_rtm)),
+               "sht_legendre_backward")
+
+    def legendre_forward(self, nvector, nscalar, WR_rtm, WS_rlm):
+        _check(lib().sht_legendre_forward(self._h, nvector, nscalar, _ptr(WR_rtm), _ptr(WS_rlm)),
+               "sht_legendre_forward")
+
+    def fft_backward(self, ncomp, WR_rtp, v_rtp):
+        _check(lib().sht_fft_backward(self._h, ncomp, _ptr(WR_rtp), _ptr(v_rtp)), "sht_fft_backward")
+
+    def fft_forward(self, ncomp, v_rtp, WS_rtp):
+        _check(lib().sht_fft_forward(self._h, ncomp, _ptr(v_rtp), _ptr(WS_rtp)), "sht_fft_forward")
+
+    def exchange_rtm_to_rtp(self, ncomp, rtm, rtp):
+        _check(lib().sht_exchange_rtm_to_rtp(self._h, ncomp, _ptr(rtm), _ptr(rtp)),
+               "sht_exchange_rtm_to_rtp")
+
+    def exchange_rtp_to_rtm(self, ncomp, rtp, rtm):
+        _check(lib().sht_exchange_rtp_to_rtm(self._h, ncomp, _ptr(rtp), _ptr(rtm)),
+               "sht_exchange_rtp_to_rtm")
+
+    def exchange_gather_rev(self, ncomp, n_dst, n_src, irev, src, dst):
+        assert irev.is_contiguous()
+        _check(lib().sht_exchange_gather_rev(self._h, ncomp, n_dst, n_src, irev.data_ptr(),
+                                             _ptr(src), _ptr(dst)), "sht_exchange_gather_rev")
+
+
+def selftest() -> int:
+    return int(lib().sht_selftest())
+
+
+# ---------------------------------------------------------------------------
+# Synthetic seeded spectra: splitmix64 -> Box-Muller, sigma(l, r) =
+# (l+1)^-p * sin(pi (r - ri)/(ro - ri)); complex c = a + i b per (l, m>=0)
+# mapped to the real harmonics as slot (l,+m) <- a, (l,-m) <- b, m = 0 takes a.
+def _splitmix64(state: np.ndarray) -> np.ndarray:
+    z = state + np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def _uniform(seed: int, n: int) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        idx = np.arange(n, dtype=np.uint64) + np.uint64(seed) * np.uint64(0x632BE59BD9B4E019)
+        bits = _splitmix64(idx)
+    return ((bits >> np.uint64(11)).astype(np.float64) + 0.5) * (1.0 / 9007199254740992.0)
+
+
+def gaussian(seed: int, n: int) -> np.ndarray:
+    u1 = _uniform(2 * seed + 1, n)
+    u2 = _uniform(2 * seed + 2, n)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def rlm_l_m(L: int):
+    """(l, m) of each j in the single-rank rlm order (m = -L..L, l = |m|..L)."""
+    ls, ms = [], []
+    for m in range(-L, L + 1):
+        for l in range(abs(m), L + 1):
+            ls.append(l)
+            ms.append(m)
+    return np.array(ls), np.array(ms)
+
+
+def synthetic_spectra(cfg: ShellConfig, nvector: int, nscalar: int, seed: int = 1,
+                      slope: float = 1.5) -> np.ndarray:
+    """Comp-interleaved rj buffer sp[nd + ncomp*(j + jmax*k)]."""
+    ncomp = 3 * nvector + nscalar
+    l, m = rlm_l_m(cfg.L)
+    r = cfg.radius()
+    prof = np.sin(np.pi * (r - cfg.ri) / (cfg.ro - cfg.ri))
+    sig = (l + 1.0) ** (-slope)
+    out = np.empty((cfg.nr, cfg.jmax, ncomp))
+    for nd in range(ncomp):
+        # one complex draw per (l, m >= 0, r): real part to +m, imaginary to -m
+        g = gaussian(seed * 1000 + nd, 2 * cfg.jmax * cfg.nr).reshape(2, cfg.nr, cfg.jmax)
+        re = g[0]
+        # the imaginary part of (l,|m|) is stored at (l,-|m|): index by the partner
+        out[:, :, nd] = np.where(m[None, :] >= 0, re, g[1]) * sig[None, :] * prof[:, None]
+    return out.ravel()

@@ -1,0 +1,185 @@
+"""GPU parity of the HIP transform engine against the CPU oracle.
+
+Bars (written in each test): reorders bit-exact; Gauss points/weights
+bit-exact; FP64 Legendre / FFT / whole transforms within relative L2 1e-12
+(per component) of the oracle's FP64 restatement of the reference."""
+import numpy as np
+import pytest
+
+import oracle as O
+from kemorin_mhd_amd import ShellConfig, SphTransPlan, selftest, synthetic_spectra
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "L7_nr3_nth12": ShellConfig(7, 3, 12, 24),
+    "cfg0_L31_nr33_nth48": ShellConfig(31, 33, 48, 96),
+    "L15_nr5_nth25_odd": ShellConfig(15, 5, 25, 40),
+    "L63_nr17_nth96": ShellConfig(63, 17, 96, 192),
+    "L20_nr2_nth33_nphi42": ShellConfig(20, 2, 33, 42),
+}
+TOL = 1e-12
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def per_comp_rel(a, b, ncomp, comp_fastest=True):
+    a = np.asarray(a).reshape(-1, ncomp) if comp_fastest else np.asarray(a).reshape(ncomp, -1).T
+    b = np.asarray(b).reshape(-1, ncomp) if comp_fastest else np.asarray(b).reshape(ncomp, -1).T
+    return max(rel_l2(a[:, c], b[:, c]) for c in range(ncomp))
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def setup(request, torch_cuda):
+    name = request.param
+    cfg = CASES[name]
+    _say(f"[gpu] case {name}")
+    plan = SphTransPlan(cfg, max_ncomp=9)
+    orc = O.OraclePlan(cfg.L, cfg.nr, cfg.nth, cfg.nphi, cfg.radius(), full_tables=False)
+    yield name, cfg, plan, orc
+    plan.close()
+
+
+def dev(t, x):
+    return t.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def test_selftest(torch_cuda):
+    _say("[gpu] selftest")
+    assert selftest() == 0
+
+
+def test_tables_bitexact(setup):
+    name, cfg, plan, orc = setup
+    assert np.array_equal(plan.table("gpoint"), orc.table("gpoint"))
+    assert np.array_equal(plan.table("weight"), orc.table("weight"))
+    assert np.array_equal(plan.table("lstack"), orc.table("lstack"))
+
+
+@pytest.mark.parametrize("nv,ns", [(2, 2), (3, 0), (0, 3), (1, 5)])
+def test_legendre_backward_stage(setup, torch_cuda, nv, ns):
+    name, cfg, plan, orc = setup
+    t = torch_cuda
+    ncomp = 3 * nv + ns
+    _say(f"[gpu] {name} legendre_backward nv={nv} ns={ns}")
+    sp = synthetic_spectra(cfg, nv, ns, seed=3)
+    ref = orc.leg_bwd(nv, ns, sp)
+    out = t.zeros(cfg.n_rtm(ncomp), dtype=t.float64, device="cuda")
+    plan.legendre_backward(nv, ns, dev(t, sp), out)
+    t.cuda.synchronize()
+    assert per_comp_rel(out.cpu().numpy(), ref, ncomp) <= TOL
+
+
+@pytest.mark.parametrize("nv,ns", [(3, 0), (2, 2), (0, 1)])
+def test_legendre_forward_stage(setup, torch_cuda, nv, ns):
+    name, cfg, plan, orc = setup
+    t = torch_cuda
+    ncomp = 3 * nv + ns
+    _say(f"[gpu] {name} legendre_forward nv={nv} ns={ns}")
+    vr = np.random.default_rng(5).standard_normal(cfg.n_rtm(ncomp))
+    ref = orc.leg_fwd(nv, ns, vr)
+    out = t.zeros(cfg.n_rj(ncomp), dtype=t.float64, device="cuda")
+    plan.legendre_forward(nv, ns, dev(t, vr), out)
+    t.cuda.synchronize()
+    assert per_comp_rel(out.cpu().numpy(), ref, ncomp) <= TOL
+
+
+def test_exchanges_bitexact(setup, torch_cuda):
+    name, cfg, plan, orc = setup
+    t = torch_cuda
+    ncomp = 5
+    _say(f"[gpu] {name} exchanges")
+    rtm = np.random.default_rng(7).standard_normal(cfg.n_rtm(ncomp))
+    ref = orc.rtm_to_rtp(ncomp, rtm)
+    rtp = t.full((cfg.n_rtp(ncomp),), 7.0, dtype=t.float64, device="cuda")
+    plan.exchange_rtm_to_rtp(ncomp, dev(t, rtm), rtp)
+    t.cuda.synchronize()
+    assert np.array_equal(rtp.cpu().numpy(), ref)
+    back = t.zeros(cfg.n_rtm(ncomp), dtype=t.float64, device="cuda")
+    plan.exchange_rtp_to_rtm(ncomp, rtp, back)
+    t.cuda.synchronize()
+    assert np.array_equal(back.cpu().numpy(), rtm)
+    # generic reverse gather with the zero dummy slot (n_src + 1)
+    n_src, n_dst = 50, 40
+    rng = np.random.default_rng(8)
+    irev = rng.integers(1, n_src + 2, size=n_dst).astype(np.int32)
+    src = rng.standard_normal(n_src * ncomp)
+    exp = np.zeros((n_dst, ncomp))
+    for i, r in enumerate(irev):
+        if r <= n_src:
+            exp[i] = src[(r - 1) * ncomp:(r) * ncomp]
+    dst = t.zeros(n_dst * ncomp, dtype=t.float64, device="cuda")
+    plan.exchange_gather_rev(ncomp, n_dst, n_src, t.from_numpy(irev).cuda(), dev(t, src), dst)
+    t.cuda.synchronize()
+    assert np.array_equal(dst.cpu().numpy(), exp.ravel())
+
+
+def test_fft_stages(setup, torch_cuda):
+    name, cfg, plan, orc = setup
+    t = torch_cuda
+    ncomp = 3
+    _say(f"[gpu] {name} fft stages")
+    rng = np.random.default_rng(9)
+    # backward input: rtp buffer with only the slots of orders m <= L filled
+    rtm = rng.standard_normal(cfg.n_rtm(ncomp))
+    rtp = orc.rtm_to_rtp(ncomp, rtm)
+    ref = orc.fft_bwd_from_rtp(ncomp, rtp)
+    v = t.zeros(cfg.n_rtp(ncomp), dtype=t.float64, device="cuda")
+    plan.fft_backward(ncomp, dev(t, rtp), v)
+    t.cuda.synchronize()
+    assert per_comp_rel(v.cpu().numpy(), ref, ncomp, comp_fastest=False) <= TOL
+    # forward: compare the slots the exchange keeps (orders m <= L)
+    g = rng.standard_normal(cfg.n_rtp(ncomp))
+    ref = orc.fft_fwd_to_rtp(ncomp, g)
+    out = t.zeros(cfg.n_rtp(ncomp), dtype=t.float64, device="cuda")
+    plan.fft_forward(ncomp, dev(t, g), out)
+    t.cuda.synchronize()
+    got_rtm = orc.rtp_to_rtm(ncomp, out.cpu().numpy())
+    ref_rtm = orc.rtp_to_rtm(ncomp, ref)
+    assert per_comp_rel(got_rtm, ref_rtm, ncomp) <= TOL
+
+
+@pytest.mark.parametrize("nv,ns", [(2, 2), (3, 0)])
+def test_whole_backward(setup, torch_cuda, nv, ns):
+    name, cfg, plan, orc = setup
+    t = torch_cuda
+    ncomp = 3 * nv + ns
+    _say(f"[gpu] {name} sht_backward nv={nv} ns={ns}")
+    sp = synthetic_spectra(cfg, nv, ns, seed=11)
+    ref = orc.backward(nv, ns, sp)
+    v = t.zeros(cfg.n_rtp(ncomp), dtype=t.float64, device="cuda")
+    plan.sph_backward_transforms(nv, ns, dev(t, sp), v)
+    t.cuda.synchronize()
+    assert per_comp_rel(v.cpu().numpy(), ref, ncomp, comp_fastest=False) <= TOL
+
+
+@pytest.mark.parametrize("nv,ns", [(3, 0), (2, 2)])
+def test_whole_forward(setup, torch_cuda, nv, ns):
+    name, cfg, plan, orc = setup
+    t = torch_cuda
+    ncomp = 3 * nv + ns
+    _say(f"[gpu] {name} sht_forward nv={nv} ns={ns}")
+    sp = synthetic_spectra(cfg, nv, ns, seed=13)
+    grid = orc.backward(nv, ns, sp)
+    ref = orc.forward(nv, ns, grid)
+    out = t.zeros(cfg.n_rj(ncomp), dtype=t.float64, device="cuda")
+    plan.sph_forward_transforms(nv, ns, dev(t, grid), out)
+    t.cuda.synchronize()
+    assert per_comp_rel(out.cpu().numpy(), ref, ncomp) <= TOL
+
+
+def test_host_entry_points(setup):
+    name, cfg, plan, orc = setup
+    _say(f"[gpu] {name} host entry points")
+    sp = synthetic_spectra(cfg, 2, 2, seed=17)
+    v = plan.backward_host(2, 2, sp)
+    assert per_comp_rel(v, orc.backward(2, 2, sp), 8, comp_fastest=False) <= TOL
+    back = plan.forward_host(2, 2, v)
+    assert per_comp_rel(back, orc.forward(2, 2, v), 8) <= TOL
