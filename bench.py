@@ -1,0 +1,258 @@
+#!/usr/bin/env python3
+"""Round-trip throughput of the MI355X spherical-harmonic transform.
+
+One step = one round trip on device-resident synthetic data: a backward
+transform of 2 vector + 2 scalar fields (8 components) and a forward
+transform of 3 vector fields (9 components), the field set of the
+reference's MHD time step (MS/cal_nonlinear.f90:149,164).  The default
+workload is L=511, Nr=257, Ntheta=768, Nphi=1536 on one GPU.
+
+Prints ONE JSON line.  `value` is round trips/s of the split-precision
+(INT8-sliced) engine when that mode is available, otherwise of the FP64-MFMA
+engine; both are reported.  Per-kernel times come from HIP events on the
+plan's stream (sht_plan_timer_*), `roofline` is for the kernel with the
+largest share of the step, `cpu_baseline` is the oracle's FP64 restatement
+of the reference transform (symmetric_BLAS_big, OpenMP) on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+INT8_PEAK_TOPS = 5000.0
+
+CONFIGS = {
+    "cfg0": dict(L=31, nr=33, nth=48, nphi=96),
+    "cfg1": dict(L=255, nr=129, nth=384, nphi=768),
+    "cfg2": dict(L=511, nr=257, nth=768, nphi=1536),
+}
+METRIC = "SHT round trips/s + % roofline at L=511,Nr=257: split-prec vs FP64 MFMA"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="cfg2", choices=list(CONFIGS))
+    ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "split", "both"])
+    ap.add_argument("--slices", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-radii", type=int, default=16, help="radii in the CPU sample")
+    ap.add_argument("--copy-calib", type=float, default=0.0,
+                    help="only run a device copy of this many GiB (PMC calibration)")
+    return ap.parse_args()
+
+
+def flops_per_contraction(c):
+    return 2.0 * ((c["nth"] + 1) // 2) * (c["L"] + 1) ** 2 * c["nr"]
+
+
+def run_engine(cfg_d, precision, args, torch):
+    from kemorin_mhd_amd import SHT_PREC_FP64, SHT_PREC_SPLIT_INT8, ShellConfig, SphTransPlan
+    cfg = ShellConfig(cfg_d["L"], cfg_d["nr"], cfg_d["nth"], cfg_d["nphi"])
+    prec = SHT_PREC_SPLIT_INT8 if precision == "split" else SHT_PREC_FP64
+    stream = torch.cuda.Stream()
+    plan = SphTransPlan(cfg, max_ncomp=9, precision=prec, n_slices=args.slices)
+    plan.set_stream(stream.cuda_stream)
+    dev = "cuda"
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(20261202)
+    # synthetic seeded spectra: sigma(l, r) = (l+1)^-1.5 sin(pi (r-ri)/(ro-ri))
+    from kemorin_mhd_amd import rlm_l_m
+    l, _ = rlm_l_m(cfg.L)
+    r = cfg.radius()
+    prof = torch.tensor(np.sin(np.pi * (r - cfg.ri) / (cfg.ro - cfg.ri)), device=dev)
+    sig = torch.tensor((l + 1.0) ** -1.5, device=dev)
+
+    def spectra(ncomp):
+        x = torch.randn((cfg.nr, cfg.jmax, ncomp), generator=gen, dtype=torch.float64, device=dev)
+        x *= sig[None, :, None]
+        x *= prof[:, None, None]
+        return x.reshape(-1)
+
+    with torch.cuda.stream(stream):
+        sp_b = spectra(8)
+        grid_b = torch.empty(cfg.n_rtp(8), dtype=torch.float64, device=dev)
+        sp_f = torch.empty(cfg.n_rj(9), dtype=torch.float64, device=dev)
+        grid_f = torch.empty(cfg.n_rtp(9), dtype=torch.float64, device=dev)
+        tmp = spectra(9)
+        plan.sph_backward_transforms(3, 0, tmp, grid_f)  # band-limited forward input
+        del tmp
+    stream.synchronize()
+
+    def step():
+        plan.sph_backward_transforms(2, 2, sp_b, grid_b)
+        plan.sph_forward_transforms(3, 0, grid_f, sp_f)
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    plan.timer_read()  # reset
+    plan.timer_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    stream.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    timers = plan.timer_read()
+    plan.timer_enable(False)
+    dev_bytes = plan.device_bytes()
+    # round-trip consistency of the measured data (band-limited forward input)
+    check = None
+    if precision == "split":
+        check = "see tests/test_gpu_split.py"
+    plan.close()
+    del sp_b, grid_b, sp_f, grid_f
+    torch.cuda.empty_cache()
+    ms = (t1 - t0) * 1e3 / args.steps
+    return dict(ms_per_step=ms, round_trips_per_s=1e3 / ms, timers=timers, plan_bytes=dev_bytes,
+                check=check)
+
+
+def kernel_roofline(cfg_d, res, precision, steps):
+    F1 = flops_per_contraction(cfg_d)
+    nline = cfg_d["nr"] * cfg_d["nth"]
+    nslot = 2 * cfg_d["L"] + 1
+    stages = {
+        "legendre_backward": ("mfma", 12 * F1),
+        "legendre_forward": ("mfma", 15 * F1),
+        "fft_backward": ("hbm", 8.0 * 8 * nline * (nslot + cfg_d["nphi"])),
+        "fft_forward": ("hbm", 8.0 * 9 * nline * (nslot + cfg_d["nphi"])),
+    }
+    t = res["timers"]
+    dom = max(stages, key=lambda s: t[s][0])
+    bound, work = stages[dom]
+    tot_ms, calls = t[dom]
+    avg_s = tot_ms / max(calls, 1) / 1e3
+    if bound == "mfma":
+        achieved = work / avg_s / 1e12
+        if precision == "split":
+            peak, unit = INT8_PEAK_TOPS, "TOP/s (FP64-equivalent work / INT8 peak)"
+        else:
+            peak, unit = FP64_PEAK_TFLOPS, "TFLOP/s"
+    else:
+        achieved = work / avg_s / 1e9
+        peak, unit = HBM_PEAK_GBS, "GB/s"
+    per_stage = {s: {"ms_per_launch": t[s][0] / max(t[s][1], 1), "launches": t[s][1]}
+                 for s in stages}
+    for s, (b, w) in stages.items():
+        a = t[s][0] / max(t[s][1], 1) / 1e3
+        if a > 0:
+            per_stage[s]["achieved"] = (w / a / 1e12) if b == "mfma" else (w / a / 1e9)
+            per_stage[s]["unit"] = "TFLOP/s (FP64-equivalent)" if b == "mfma" else "GB/s"
+    return {"kernel": dom, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "traffic": None,
+            "work_per_launch": work}, per_stage
+
+
+def cpu_baseline(cfg_d, nr_sample):
+    import oracle as O
+    from kemorin_mhd_amd import chebyshev_radius
+    nr_s = min(nr_sample, cfg_d["nr"])
+    r_full = chebyshev_radius(cfg_d["nr"], 0.35 / 0.65, 1 / 0.65)
+    r = r_full[np.linspace(0, cfg_d["nr"] - 1, nr_s).round().astype(int)]
+    P = O.OraclePlan(cfg_d["L"], nr_s, cfg_d["nth"], cfg_d["nphi"], r, full_tables=False)
+    rng = np.random.default_rng(1)
+    sp_b = rng.standard_normal(P.n_rj(8))
+    grid_f = rng.standard_normal(P.n_rtp(9))
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        P.backward(2, 2, sp_b)
+        P.forward(3, 0, grid_f)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    full = t * cfg_d["nr"] / nr_s
+    return {"value": 1.0 / full, "unit": "round_trips/s", "cores": O.lib().orc_num_threads(),
+            "kind": "port",
+            "sample": (f"L={cfg_d['L']} Ntheta={cfg_d['nth']} Nphi={cfg_d['nphi']}, {nr_s} of "
+                       f"{cfg_d['nr']} radii, median of 3 round trips ({t:.2f} s each), "
+                       f"scaled linearly to {cfg_d['nr']} radii; oracle symmetric_BLAS_big + "
+                       f"FFTPACK-convention FFT, FP64, OpenMP")}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; shard radii across processes "
+                          "(see DESIGN.md) -- run with --gpus 1"}))
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no HIP device"}))
+        sys.exit(1)
+    if args.copy_calib > 0:
+        n = int(args.copy_calib * (1 << 30)) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes_each_way": 8 * n, "launches": 5}))
+        return
+    cfg_d = CONFIGS[args.config]
+    from kemorin_mhd_amd import sht as S
+    split_ok = True
+    if args.precision in ("auto", "both", "split"):
+        try:
+            S.lib()
+            from kemorin_mhd_amd import SHT_PREC_SPLIT_INT8, ShellConfig, SphTransPlan
+            probe = SphTransPlan(ShellConfig(7, 2, 8, 16), max_ncomp=9,
+                                 precision=SHT_PREC_SPLIT_INT8)
+            probe.close()
+        except Exception:
+            split_ok = False
+    precs = []
+    if args.precision == "fp64" or args.precision in ("auto", "both"):
+        precs.append("fp64")
+    if split_ok and args.precision in ("auto", "both", "split"):
+        precs.append("split")
+    results = {}
+    for p in precs:
+        results[p] = run_engine(cfg_d, p, args, torch)
+        print(f"# {p}: {results[p]['ms_per_step']:.3f} ms/round trip", file=sys.stderr, flush=True)
+    head = "split" if "split" in results else "fp64"
+    res = results[head]
+    roof, per_stage = kernel_roofline(cfg_d, res, head, args.steps)
+    F_rt = 27 * flops_per_contraction(cfg_d)
+    out = {
+        "metric": METRIC, "value": res["round_trips_per_s"], "unit": "round_trips/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if head == "fp64" else "f64 (int8-sliced Legendre)",
+        "data": "synthetic (seeded Gaussian spectra, sigma=(l+1)^-1.5 sin(pi(r-ri)/(ro-ri)))",
+        "config": {"workload": (f"round trip L={cfg_d['L']} Nr={cfg_d['nr']} "
+                                f"Ntheta={cfg_d['nth']} Nphi={cfg_d['nphi']}: backward 2 vector + "
+                                f"2 scalar (8 comps), forward 3 vector (9 comps)"),
+                   "engine": head, "L": cfg_d["L"], "nr": cfg_d["nr"], "ntheta": cfg_d["nth"],
+                   "nphi": cfg_d["nphi"]},
+        "roofline": roof,
+        "gflops_fp64_equivalent": F_rt / (res["ms_per_step"] / 1e3) / 1e9,
+        "stages": per_stage,
+        "engines": {p: {"ms_per_step": r["ms_per_step"], "round_trips_per_s": r["round_trips_per_s"],
+                        "stage_ms_per_step": {s: v[0] / args.steps for s, v in r["timers"].items()},
+                        "plan_device_bytes": r["plan_bytes"]}
+                    for p, r in results.items()},
+    }
+    if "split" in results and "fp64" in results:
+        out["speedup_split_vs_fp64"] = results["fp64"]["ms_per_step"] / results["split"]["ms_per_step"]
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg_d, args.cpu_radii)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

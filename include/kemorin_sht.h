@@ -71,6 +71,17 @@ int sht_plan_get_table(const sht_plan *plan, const char *name, void *out, long l
 /* Device bytes held by the plan (tables + workspace). */
 long long sht_plan_device_bytes(const sht_plan *plan);
 
+/* Stage timers, the counterpart of the reference's elapsed-time ids for the
+ * transform (MHD/Fortran_src/MHD_src/IO/init_sph_MHD_elapsed_label.f90:59-65:
+ * 22 Legendre backward, 23 Legendre forward, 24 FFT, 18-21 exchanges).
+ * When enabled, every kernel launch of the plan is bracketed by HIP events
+ * on the plan's stream.  Stage ids: 0 Legendre backward, 1 FFT backward,
+ * 2 FFT forward, 3 Legendre forward, 4 exchanges.
+ * sht_plan_timer_read synchronises, returns per-stage total milliseconds and
+ * launch counts (n entries each, n <= 5) and resets the counters. */
+int sht_plan_timer_enable(sht_plan *plan, int on);
+int sht_plan_timer_read(sht_plan *plan, double *ms, long long *calls, int n);
+
 /* Whole transforms on device-resident buffers.
  * Replaces sph_backward_transforms (SS/t_sph_transforms.f90:74-123):
  * d_WS_rj is the rj->rlm send buffer (spectral layout above, ncomp =

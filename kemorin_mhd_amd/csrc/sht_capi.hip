@@ -72,11 +72,13 @@ static FourierAddr rtp_fourier_addr(const sht_plan *p, int ncomp) {
 
 static int leg_bwd(sht_plan *p, int nv, int ns, const double *sp, SpecAddr sa, double *out,
                    GridAddr ga) {
+  StageTimer tm(p, 0);
   if (p->precision == SHT_PREC_SPLIT_INT8) return launch_leg_bwd_split(p, nv, ns, sp, sa, out, ga);
   return launch_leg_bwd(p, nv, ns, sp, sa, out, ga);
 }
 static int leg_fwd(sht_plan *p, int nv, int ns, const double *in, GridAddr ga, double *sp,
                    SpecAddr sa) {
+  StageTimer tm(p, 3);
   if (p->precision == SHT_PREC_SPLIT_INT8) return launch_leg_fwd_split(p, nv, ns, in, ga, sp, sa);
   return launch_leg_fwd(p, nv, ns, in, ga, sp, sa);
 }
@@ -87,6 +89,7 @@ extern "C" int sht_backward(sht_plan *p, int nv, int ns, const double *WS_rj, do
   const int ncomp = 3 * nv + ns;
   rc = leg_bwd(p, nv, ns, WS_rj, spec_addr(p, ncomp), p->X, x_grid_addr(p));
   if (rc) return rc;
+  StageTimer tm(p, 1);
   return launch_fft_bwd(p, ncomp, p->X, x_fourier_addr(p), v_rtp);
 }
 
@@ -94,7 +97,10 @@ extern "C" int sht_forward(sht_plan *p, int nv, int ns, const double *v_rtp, dou
   int rc = check_call(p, nv, ns, v_rtp, WR_rj, "sht_forward");
   if (rc) return rc;
   const int ncomp = 3 * nv + ns;
-  rc = launch_fft_fwd(p, ncomp, v_rtp, p->X, x_fourier_addr(p));
+  {
+    StageTimer tm(p, 2);
+    rc = launch_fft_fwd(p, ncomp, v_rtp, p->X, x_fourier_addr(p));
+  }
   if (rc) return rc;
   return leg_fwd(p, nv, ns, p->X, x_grid_addr(p), WR_rj, spec_addr(p, ncomp));
 }
@@ -150,6 +156,7 @@ extern "C" int sht_legendre_forward(sht_plan *p, int nv, int ns, const double *W
 extern "C" int sht_fft_backward(sht_plan *p, int ncomp, const double *WR_rtp, double *v_rtp) {
   int rc = check_call(p, 0, ncomp, WR_rtp, v_rtp, "sht_fft_backward");
   if (rc) return rc;
+  StageTimer tm(p, 1);
   return launch_fft_bwd(p, ncomp, WR_rtp, rtp_fourier_addr(p, ncomp), v_rtp);
 }
 
@@ -157,6 +164,7 @@ extern "C" int sht_fft_forward(sht_plan *p, int ncomp, const double *v_rtp, doub
   int rc = check_call(p, 0, ncomp, v_rtp, WS_rtp, "sht_fft_forward");
   if (rc) return rc;
   const size_t bytes = sizeof(double) * (size_t)ncomp * p->nr * p->nth * p->nphi;
+  StageTimer tm(p, 2);
   KSHT_HIP(hipMemsetAsync(WS_rtp, 0, bytes, p->stream));
   return launch_fft_fwd(p, ncomp, v_rtp, WS_rtp, rtp_fourier_addr(p, ncomp));
 }
@@ -164,12 +172,14 @@ extern "C" int sht_fft_forward(sht_plan *p, int ncomp, const double *v_rtp, doub
 extern "C" int sht_exchange_rtm_to_rtp(sht_plan *p, int ncomp, const double *rtm, double *rtp) {
   int rc = check_call(p, 0, ncomp, rtm, rtp, "sht_exchange_rtm_to_rtp");
   if (rc) return rc;
+  StageTimer tm(p, 4);
   return launch_rtm_to_rtp(p, ncomp, rtm, rtp);
 }
 
 extern "C" int sht_exchange_rtp_to_rtm(sht_plan *p, int ncomp, const double *rtp, double *rtm) {
   int rc = check_call(p, 0, ncomp, rtp, rtm, "sht_exchange_rtp_to_rtm");
   if (rc) return rc;
+  StageTimer tm(p, 4);
   return launch_rtp_to_rtm(p, ncomp, rtp, rtm);
 }
 
@@ -180,7 +190,43 @@ extern "C" int sht_exchange_gather_rev(sht_plan *p, int ncomp, long long n_dst, 
     set_error("sht_exchange_gather_rev: bad arguments");
     return SHT_ERR_ARG;
   }
+  StageTimer tm(p, 4);
   return launch_gather_rev(p, ncomp, n_dst, n_src, irev, src, dst);
+}
+
+extern "C" int sht_plan_timer_enable(sht_plan *p, int on) {
+  if (!p) {
+    set_error("sht_plan_timer_enable: null plan");
+    return SHT_ERR_ARG;
+  }
+  p->timing = on ? 1 : 0;
+  return SHT_OK;
+}
+
+extern "C" int sht_plan_timer_read(sht_plan *p, double *ms, long long *calls, int n) {
+  if (!p || n < 0 || n > kNumStages || (n > 0 && (!ms || !calls))) {
+    set_error("sht_plan_timer_read: bad arguments");
+    return SHT_ERR_ARG;
+  }
+  KSHT_HIP(hipStreamSynchronize(p->stream));
+  int rc = SHT_OK;
+  for (int s = 0; s < kNumStages; s++) {
+    double tot = 0.0;
+    for (auto &e : p->tev[s]) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, e.first, e.second) != hipSuccess) rc = SHT_ERR_HIP;
+      tot += t;
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    if (s < n) {
+      ms[s] = tot;
+      calls[s] = (long long)p->tev[s].size();
+    }
+    p->tev[s].clear();
+  }
+  if (rc) set_error("sht_plan_timer_read: event query failed");
+  return rc;
 }
 
 // ---------------------------------------------------------------------------

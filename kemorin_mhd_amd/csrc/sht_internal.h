@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/kemorin_sht.h"
@@ -17,6 +18,7 @@ typedef int i4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTile = 16;   // MFMA tile edge (rows, radii, degrees)
 constexpr int kMaxPass = 24;
+constexpr int kNumStages = 5;  // Legendre bwd, FFT bwd, FFT fwd, Legendre fwd, exchanges
 
 void set_error(const char *fmt, ...);
 
@@ -96,7 +98,31 @@ struct sht_plan {
   long long device_bytes = 0;
   // split-precision state (allocated only in SHT_PREC_SPLIT_INT8 mode)
   void *split = nullptr;
+  // stage timers (sht_plan_timer_*): event pairs recorded per launch
+  int timing = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> tev[ksht::kNumStages];
 };
+
+namespace ksht {
+// Brackets one stage launch with events when the plan's timers are on.
+struct StageTimer {
+  sht_plan *p;
+  int id;
+  hipEvent_t a = nullptr, b = nullptr;
+  StageTimer(sht_plan *p_, int id_) : p(p_), id(id_) {
+    if (p->timing && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+      (void)hipEventRecord(a, p->stream);
+  }
+  ~StageTimer() {
+    if (b) {
+      (void)hipEventRecord(b, p->stream);
+      p->tev[id].push_back({a, b});
+    } else if (a) {
+      (void)hipEventDestroy(a);
+    }
+  }
+};
+}  // namespace ksht
 
 namespace ksht {
 // Legendre launchers (sht_legendre.hip)

@@ -230,6 +230,11 @@ static void free_plan(sht_plan *p) {
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   split_free(p);
+  for (auto &v : p->tev)
+    for (auto &e : v) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
   if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
   delete p;
 }

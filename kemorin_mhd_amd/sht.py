@@ -67,6 +67,9 @@ def lib() -> ctypes.CDLL:
             getattr(L, nm).argtypes = [_vp, ctypes.c_int, _vp, _vp]
         L.sht_exchange_gather_rev.argtypes = [_vp, ctypes.c_int, ctypes.c_longlong,
                                               ctypes.c_longlong, _vp, _vp, _vp]
+        L.sht_plan_timer_enable.argtypes = [_vp, ctypes.c_int]
+        L.sht_plan_timer_read.argtypes = [_vp, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
         L.sht_get_last_error.argtypes = [ctypes.c_char_p, ctypes.c_int]
         L.sht_version.restype = ctypes.c_char_p
         _LIB = L
@@ -170,6 +173,18 @@ class SphTransPlan:
 
     def set_stream(self, stream_ptr: int | None):
         _check(lib().sht_plan_set_stream(self._h, stream_ptr), "sht_plan_set_stream")
+
+    STAGES = ("legendre_backward", "fft_backward", "fft_forward", "legendre_forward", "exchange")
+
+    def timer_enable(self, on: bool = True):
+        _check(lib().sht_plan_timer_enable(self._h, int(on)), "sht_plan_timer_enable")
+
+    def timer_read(self) -> dict:
+        """{stage: (total_ms, launches)} since the last read (synchronises)."""
+        ms = (ctypes.c_double * 5)()
+        calls = (ctypes.c_longlong * 5)()
+        _check(lib().sht_plan_timer_read(self._h, ms, calls, 5), "sht_plan_timer_read")
+        return {s: (ms[i], int(calls[i])) for i, s in enumerate(self.STAGES)}
 
     def device_bytes(self) -> int:
         return int(lib().sht_plan_device_bytes(self._h))
