@@ -16,7 +16,18 @@
 namespace ksht {
 
 constexpr int kFftThreads = 256;
-constexpr int kFftElems = 6144;  // complex elements per work-group in LDS
+// Complex elements per work-group held in LDS: 3072 (48 KiB, three
+// work-groups per CU) for Nphi <= 3072, 6144 (96 KiB) for longer lines.
+constexpr int kFftElemsSmall = 3072, kFftElemsLarge = 6144;
+
+// Work-groups are dispatched round-robin over the 8 XCDs; neighbouring line
+// blocks share 128-byte lines of the phi-major grid, so consecutive logical
+// blocks are kept on one XCD (and its L2).
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+  const unsigned per = nb / 8;
+  if (b >= per * 8) return b;  // tail blocks keep their index
+  return (b % 8) * per + b / 8;
+}
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -84,10 +95,10 @@ __device__ __forceinline__ void dft_small(double2 *v, double s) {
 }
 
 // One Stockham pass over F complex FFTs of length n held in LDS.
-template <int R>
+template <int E, int R>
 __device__ void fft_pass(double2 *buf, int n, int F, int Ns, const double2 *__restrict__ tw,
                          double s) {
-  constexpr int NBT = (kFftElems / kFftThreads + R - 1) / R;
+  constexpr int NBT = (E / kFftThreads + R - 1) / R;
   const int nR = n / R, nb = F * nR;
   double2 v[NBT][R];
 #pragma unroll
@@ -123,33 +134,42 @@ __device__ void fft_pass(double2 *buf, int n, int F, int Ns, const double2 *__re
   __syncthreads();
 }
 
+template <int E, bool GEN>
 __device__ void fft_run(double2 *buf, const FftPlan &fp, int F, const double2 *__restrict__ tw,
                         double s) {
   for (int ip = 0; ip < fp.npass; ip++) {
     const FftPass ps = fp.pass[ip];
     const double2 *t = tw + ps.tw_off;
     switch (ps.R) {
-      case 8: fft_pass<8>(buf, fp.n, F, ps.Ns, t, s); break;
-      case 4: fft_pass<4>(buf, fp.n, F, ps.Ns, t, s); break;
-      case 2: fft_pass<2>(buf, fp.n, F, ps.Ns, t, s); break;
-      case 3: fft_pass<3>(buf, fp.n, F, ps.Ns, t, s); break;
-      case 5: fft_pass<5>(buf, fp.n, F, ps.Ns, t, s); break;
-      case 7: fft_pass<7>(buf, fp.n, F, ps.Ns, t, s); break;
-      case 11: fft_pass<11>(buf, fp.n, F, ps.Ns, t, s); break;
-      case 13: fft_pass<13>(buf, fp.n, F, ps.Ns, t, s); break;
-      default: break;
+      case 8: fft_pass<E, 8>(buf, fp.n, F, ps.Ns, t, s); break;
+      case 4: fft_pass<E, 4>(buf, fp.n, F, ps.Ns, t, s); break;
+      case 2: fft_pass<E, 2>(buf, fp.n, F, ps.Ns, t, s); break;
+      case 3: fft_pass<E, 3>(buf, fp.n, F, ps.Ns, t, s); break;
+      default:
+        // radices 5, 7, 11, 13 only in the GEN instantiation (register cost)
+        if constexpr (GEN) {
+          switch (ps.R) {
+            case 5: fft_pass<E, 5>(buf, fp.n, F, ps.Ns, t, s); break;
+            case 7: fft_pass<E, 7>(buf, fp.n, F, ps.Ns, t, s); break;
+            case 11: fft_pass<E, 11>(buf, fp.n, F, ps.Ns, t, s); break;
+            case 13: fft_pass<E, 13>(buf, fp.n, F, ps.Ns, t, s); break;
+            default: break;
+          }
+        }
+        break;
     }
   }
 }
 
 // Fourier coefficients -> grid.  blockIdx.x: block of B lines, blockIdx.y: nd.
+template <int E, bool GEN>
 __global__ void __launch_bounds__(kFftThreads) k_fft_bwd(
     FftPlan fp, int L, long long nline, const double *__restrict__ X, FourierAddr xa,
     double *__restrict__ v, const double2 *__restrict__ tw) {
   extern __shared__ double2 buf[];
   const int n = fp.n, B = fp.lines_per_wg, F = B / 2;
   const int nd = blockIdx.y;
-  const long long line0 = (long long)blockIdx.x * B;
+  const long long line0 = (long long)xcd_block(blockIdx.x, gridDim.x) * B;
   const long long nnod = nline * n;
   // zero the orders above L
   for (int idx = threadIdx.x; idx < F * (n - 2 * L - 1); idx += kFftThreads) {
@@ -183,7 +203,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_bwd(
     }
   }
   __syncthreads();
-  fft_run(buf, fp, F, tw, +1.0);
+  fft_run<E, GEN>(buf, fp, F, tw, +1.0);
   for (int idx = threadIdx.x; idx < F * n; idx += kFftThreads) {
     const int j = idx / F, f = idx - j * F;
     const long long la = line0 + 2 * f;
@@ -203,13 +223,14 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_bwd(
 }
 
 // grid -> Fourier coefficients (orders m <= L).
+template <int E, bool GEN>
 __global__ void __launch_bounds__(kFftThreads) k_fft_fwd(
     FftPlan fp, int L, long long nline, const double *__restrict__ v, double *__restrict__ X,
     FourierAddr xa, const double2 *__restrict__ tw) {
   extern __shared__ double2 buf[];
   const int n = fp.n, B = fp.lines_per_wg, F = B / 2;
   const int nd = blockIdx.y;
-  const long long line0 = (long long)blockIdx.x * B;
+  const long long line0 = (long long)xcd_block(blockIdx.x, gridDim.x) * B;
   const long long nnod = nline * n;
   for (int idx = threadIdx.x; idx < F * n; idx += kFftThreads) {
     const int j = idx / F, f = idx - j * F;
@@ -227,7 +248,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_fwd(
     buf[f * n + j] = z;
   }
   __syncthreads();
-  fft_run(buf, fp, F, tw, -1.0);
+  fft_run<E, GEN>(buf, fp, F, tw, -1.0);
   const double sn = 1.0 / (double)n, tsn = 2.0 / (double)n;
   for (int idx = threadIdx.x; idx < F * (L + 1); idx += kFftThreads) {
     const int m = idx / F, f = idx - m * F;
@@ -273,11 +294,13 @@ int fft_plan_init(sht_plan *p) {
     set_error("FFT length %d has a prime factor > 13 (unsupported)", n);
     return SHT_ERR_UNSUPPORTED;
   }
-  int F = kFftElems / n;  // complex FFTs per work-group
+  const int E = n <= kFftElemsSmall ? kFftElemsSmall : kFftElemsLarge;
+  int F = E / n;  // complex FFTs per work-group
   if (F < 1) {
-    set_error("FFT length %d exceeds the LDS capacity (%d)", n, kFftElems);
+    set_error("FFT length %d exceeds the LDS capacity (%d)", n, kFftElemsLarge);
     return SHT_ERR_UNSUPPORTED;
   }
+  fp.pad = E;
   if (F > 32) F = 32;
   int f2 = 1;
   while (f2 * 2 <= F) f2 *= 2;
@@ -300,8 +323,16 @@ int fft_plan_init(sht_plan *p) {
     Ns *= R;
   }
   const int lds = (int)(sizeof(double2) * (size_t)f2 * n);
-  KSHT_HIP(hipFuncSetAttribute((const void *)k_fft_bwd, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  KSHT_HIP(hipFuncSetAttribute((const void *)k_fft_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  bool gen = false;
+  for (int R : fac) gen |= !(R == 2 || R == 3 || R == 4 || R == 8);
+  p->fft_generic = gen;
+  const void *kb[4] = {(const void *)k_fft_bwd<kFftElemsSmall, false>, (const void *)k_fft_bwd<kFftElemsSmall, true>,
+                       (const void *)k_fft_bwd<kFftElemsLarge, false>, (const void *)k_fft_bwd<kFftElemsLarge, true>};
+  const void *kf[4] = {(const void *)k_fft_fwd<kFftElemsSmall, false>, (const void *)k_fft_fwd<kFftElemsSmall, true>,
+                       (const void *)k_fft_fwd<kFftElemsLarge, false>, (const void *)k_fft_fwd<kFftElemsLarge, true>};
+  const int sel = (E == kFftElemsSmall ? 0 : 2) + (gen ? 1 : 0);
+  KSHT_HIP(hipFuncSetAttribute(kb[sel], hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  KSHT_HIP(hipFuncSetAttribute(kf[sel], hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   KSHT_HIP(hipMalloc(&p->t.tw, sizeof(double2) * tw.size()));
   KSHT_HIP(hipMemcpy(p->t.tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice));
   p->device_bytes += (long long)(sizeof(double2) * tw.size());
@@ -313,8 +344,21 @@ int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, doub
   const int B = p->fft.lines_per_wg;
   const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);
   const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.n;
-  hipLaunchKernelGGL(k_fft_bwd, grid, dim3(kFftThreads), lds, p->stream, p->fft, p->L, nline, X,
-                     xa, v, p->t.tw);
+#define KSHT_FFT_LAUNCH(K, ...)                                                              \
+  do {                                                                                     \
+    if (p->fft.pad == kFftElemsSmall) {                                                    \
+      if (p->fft_generic)                                                                  \
+        hipLaunchKernelGGL((K<kFftElemsSmall, true>), grid, dim3(kFftThreads), lds, p->stream, __VA_ARGS__); \
+      else                                                                                 \
+        hipLaunchKernelGGL((K<kFftElemsSmall, false>), grid, dim3(kFftThreads), lds, p->stream, __VA_ARGS__); \
+    } else {                                                                               \
+      if (p->fft_generic)                                                                  \
+        hipLaunchKernelGGL((K<kFftElemsLarge, true>), grid, dim3(kFftThreads), lds, p->stream, __VA_ARGS__); \
+      else                                                                                 \
+        hipLaunchKernelGGL((K<kFftElemsLarge, false>), grid, dim3(kFftThreads), lds, p->stream, __VA_ARGS__); \
+    }                                                                                      \
+  } while (0)
+  KSHT_FFT_LAUNCH(k_fft_bwd, p->fft, p->L, nline, X, xa, v, p->t.tw);
   KSHT_HIP(hipGetLastError());
   return SHT_OK;
 }
@@ -324,8 +368,7 @@ int launch_fft_fwd(sht_plan *p, int ncomp, const double *v, double *X, FourierAd
   const int B = p->fft.lines_per_wg;
   const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);
   const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.n;
-  hipLaunchKernelGGL(k_fft_fwd, grid, dim3(kFftThreads), lds, p->stream, p->fft, p->L, nline, v,
-                     X, xa, p->t.tw);
+  KSHT_FFT_LAUNCH(k_fft_fwd, p->fft, p->L, nline, v, X, xa, p->t.tw);
   KSHT_HIP(hipGetLastError());
   return SHT_OK;
 }

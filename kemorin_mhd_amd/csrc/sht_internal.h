@@ -38,7 +38,7 @@ struct FftPass {
   int R, Ns, tw_off, pad;
 };
 struct FftPlan {
-  int n, npass, lines_per_wg, pad;
+  int n, npass, lines_per_wg, pad;  // pad: LDS elements per work-group
   FftPass pass[kMaxPass];
 };
 
@@ -93,6 +93,7 @@ struct sht_plan {
   long long p_table_elems_b, p_table_elems_f;
   ksht::Tables t;
   ksht::FftPlan fft;
+  bool fft_generic = false;  // FFT needs radices other than 2, 3, 4, 8
   double *X = nullptr;        // field-major Fourier workspace
   long long x_elems = 0;
   long long device_bytes = 0;

@@ -44,23 +44,28 @@ struct LegDims {
 // ---------------------------------------------------------------------------
 // Backward (spectra -> Legendre-side grid).  Work-group = (M, radius tile,
 // component group, theta block); 4 waves, wave w owns hemisphere rows
-// [16*(4*tb+w), +16).  Group cg < nv: vector cg; otherwise scalars
-// 4*(cg-nv) .. +3.
-constexpr int kVecQ = 5;  // P*pol*g3, P*tor, P*dpol, dP*dpol, dP*tor
+// [16*(4*tb+w), +16).  VEC: one vector (pol, dpol, tor); otherwise up to 4
+// scalars.  Software pipeline: the spectra of degree chunk ic+1 (32 degrees,
+// 16 per parity) and the P/dP fragments of chunk ic+1 are loaded into
+// registers while the MFMAs of chunk ic run on the other LDS buffer.
 constexpr int kSclPer = 4;
 
+template <bool VEC>
 __global__ void __launch_bounds__(256) k_leg_bwd(
-    LegDims d, const double *__restrict__ sp, SpecAddr sa, double *__restrict__ out,
+    LegDims d, int cg0, const double *__restrict__ sp, SpecAddr sa, double *__restrict__ out,
     GridAddr ga, const int *__restrict__ jst, const long long *__restrict__ offb,
     const int *__restrict__ ipad, const double *__restrict__ Pb,
     const double *__restrict__ dPb, const double *__restrict__ a_r,
     const double *__restrict__ asin_t) {
-  __shared__ double sB[2 * 4 * 2 * 16 * 16];  // [sig][c][par][ii][kk]
+  constexpr int NC = VEC ? 3 : kSclPer;  // components per group
+  constexpr int NQ = VEC ? 5 : kSclPer;  // P*pol*g3, P*tor, P*dpol, dP*dpol, dP*tor
+  constexpr int SBUF = 2 * NC * 2 * 256;
+  __shared__ double sB[2][SBUF];  // [buf][sig][c][par][ii][kk]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   long long b = blockIdx.x;
   const int tb = (int)(b % d.nblk);
   b /= d.nblk;
-  const int cg = (int)(b % d.ncg);
+  const int cgl = (int)(b % d.ncg);
   b /= d.ncg;
   const int rt = (int)(b % d.n_rt);
   const int M = (int)(b / d.n_rt);
@@ -70,76 +75,112 @@ __global__ void __launch_bounds__(256) k_leg_bwd(
   const int k0 = rt * 16;
   const int t0 = (tb * 4 + wave) * 16;
   const bool active = t0 < d.nth_h;
-  const bool is_vec = cg < d.nv;
-  const int nc = is_vec ? 3 : min(kSclPer, d.ns - kSclPer * (cg - d.nv));
-  const int nd0 = is_vec ? 3 * cg : 3 * d.nv + kSclPer * (cg - d.nv);
-  const int ip[2] = {ipad[2 * M], ipad[2 * M + 1]};
-  const long long ob[2] = {offb[2 * M], offb[2 * M + 1]};
-  const int slot[2] = {L + M, L - M};
-  const int jb[2] = {jst[L + M], jst[L - M]};
+  const int cg = cg0 + cgl;
+  const int nc = VEC ? 3 : min(kSclPer, d.ns - kSclPer * (cg - d.nv));
+  const int nd0 = VEC ? 3 * cg : 3 * d.nv + kSclPer * (cg - d.nv);
+  const int ip0 = ipad[2 * M], ip1 = ipad[2 * M + 1];
+  const long long ob0 = offb[2 * M], ob1 = offb[2 * M + 1];
+  const int jb0 = jst[L + M], jb1 = jst[L - M];
 
-  d4 acc[2][2][kVecQ];
+  d4 acc[2][2][NQ];
 #pragma unroll
   for (int s = 0; s < 2; s++)
 #pragma unroll
     for (int p = 0; p < 2; p++)
 #pragma unroll
-      for (int q = 0; q < kVecQ; q++) acc[s][p][q] = d4{0.0, 0.0, 0.0, 0.0};
+      for (int q = 0; q < NQ; q++) acc[s][p][q] = d4{0.0, 0.0, 0.0, 0.0};
 
-  const int nchunk = (ne + 15) / 16;
-  for (int ic = 0; ic < nchunk; ic++) {
-    __syncthreads();
-    for (int idx = tid; idx < 2 * 16 * 32; idx += 256) {
+  // staging: 1024 items (sig, degree offset lo, radius kk), 4 per thread
+  double sreg[4][NC];
+  auto stage_load = [&](int ic) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int idx = tid + 256 * u;
       const int lo = idx & 31, kk = (idx >> 5) & 15, sg = idx >> 9;
-      const int par = lo & 1, ii = lo >> 1;
-      const int l = M + 2 * (ic * 16 + ii) + par;
+      const int l = M + 2 * (ic * 16 + (lo >> 1)) + (lo & 1);
       const int k = k0 + kk;
       const bool valid = sg < nsig && l <= L && k < d.nr;
-      const int jbs = sg == 0 ? jb[0] : jb[1];
+      const int jbs = sg == 0 ? jb0 : jb1;
       const double *src = sp + (long long)(jbs + l - M) * sa.s_j + (long long)k * sa.s_k +
                           (long long)nd0 * sa.s_nd;
-      double *dst = sB + ((sg * 4 * 2 + par) * 16 + ii) * 16 + kk;  // + c*512
-      if (is_vec) {
-        const double pol = valid ? src[0] * g3_of(l) : 0.0;
-        const double dpol = valid ? src[sa.s_nd] : 0.0;
-        const double tor = valid ? src[2 * sa.s_nd] : 0.0;
-        dst[0 * 512] = pol;
-        dst[1 * 512] = dpol;
-        dst[2 * 512] = tor;
-      } else {
-        for (int c = 0; c < kSclPer; c++)
-          dst[c * 512] = (valid && c < nc) ? src[(long long)c * sa.s_nd] : 0.0;
-      }
+#pragma unroll
+      for (int c = 0; c < NC; c++)
+        sreg[u][c] = (valid && c < nc) ? src[(long long)c * sa.s_nd] : 0.0;
+      if (VEC) sreg[u][0] *= g3_of(l);
     }
-    __syncthreads();
-    if (!active) continue;
+  };
+  auto stage_store = [&](double *buf) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int idx = tid + 256 * u;
+      const int lo = idx & 31, kk = (idx >> 5) & 15, sg = idx >> 9;
+      const int par = lo & 1, ii = lo >> 1;
+#pragma unroll
+      for (int c = 0; c < NC; c++) buf[(((sg * NC + c) * 2 + par) * 16 + ii) * 16 + kk] = sreg[u][c];
+    }
+  };
+  double pc[4][2], dpc[4][2], pn[4][2], dpn[4][2];
+  auto pload = [&](int ic, double (&a)[4][2], double (&da)[4][2]) {
 #pragma unroll
     for (int ks = 0; ks < 4; ks++) {
-      const int ii = ks * 4 + (lane >> 4);
-      const int ia = ic * 16 + ii;
+      const long long row = (long long)(ic * 16 + ks * 4 + (lane >> 4)) * d.nth_hp + t0 + (lane & 15);
+      const bool v0 = ic * 16 < ip0, v1 = ic * 16 < ip1;
+      a[ks][0] = v0 ? Pb[ob0 + row] : 0.0;
+      a[ks][1] = v1 ? Pb[ob1 + row] : 0.0;
+      da[ks][0] = (VEC && v0) ? dPb[ob0 + row] : 0.0;
+      da[ks][1] = (VEC && v1) ? dPb[ob1 + row] : 0.0;
+    }
+  };
+
+  const int nchunk = (ne + 15) / 16;
+  if (nchunk > 0) {
+    stage_load(0);
+    if (active) pload(0, pc, dpc);
+  }
+  for (int ic = 0; ic < nchunk; ic++) {
+    double *buf = sB[ic & 1];
+    stage_store(buf);
+    __syncthreads();
+    const bool more = ic + 1 < nchunk;
+    if (more) {
+      stage_load(ic + 1);
+      if (active) pload(ic + 1, pn, dpn);
+    }
+    if (active) {
 #pragma unroll
-      for (int par = 0; par < 2; par++) {
-        if (ic * 16 >= ip[par]) continue;
-        const long long ao = ob[par] + (long long)ia * d.nth_hp + t0 + (lane & 15);
-        const double aP = Pb[ao];
-        const double adP = is_vec ? dPb[ao] : 0.0;
+      for (int ks = 0; ks < 4; ks++) {
+        const int ii = ks * 4 + (lane >> 4);
 #pragma unroll
-        for (int sg = 0; sg < 2; sg++) {
-          if (sg >= nsig) continue;
-          const double *bb = sB + ((sg * 4 * 2 + par) * 16 + ii) * 16 + (lane & 15);
-          if (is_vec) {
-            const double bpol = bb[0], bdpol = bb[512], btor = bb[1024];
-            acc[sg][par][0] = mfma64(aP, bpol, acc[sg][par][0]);
-            acc[sg][par][1] = mfma64(aP, btor, acc[sg][par][1]);
-            acc[sg][par][2] = mfma64(aP, bdpol, acc[sg][par][2]);
-            acc[sg][par][3] = mfma64(adP, bdpol, acc[sg][par][3]);
-            acc[sg][par][4] = mfma64(adP, btor, acc[sg][par][4]);
-          } else {
+        for (int par = 0; par < 2; par++) {
+          if (ic * 16 >= (par ? ip1 : ip0)) continue;
+          const double aP = pc[ks][par], adP = dpc[ks][par];
 #pragma unroll
-            for (int c = 0; c < kSclPer; c++) acc[sg][par][c] = mfma64(aP, bb[c * 512], acc[sg][par][c]);
+          for (int sg = 0; sg < 2; sg++) {
+            if (sg >= nsig) continue;
+            const double *bb = buf + ((sg * NC * 2 + par) * 16 + ii) * 16 + (lane & 15);
+            if (VEC) {
+              const double bpol = bb[0], bdpol = bb[512], btor = bb[1024];
+              acc[sg][par][0] = mfma64(aP, bpol, acc[sg][par][0]);
+              acc[sg][par][1] = mfma64(aP, btor, acc[sg][par][1]);
+              acc[sg][par][2] = mfma64(aP, bdpol, acc[sg][par][2]);
+              acc[sg][par][3] = mfma64(adP, bdpol, acc[sg][par][3]);
+              acc[sg][par][4] = mfma64(adP, btor, acc[sg][par][4]);
+            } else {
+#pragma unroll
+              for (int c = 0; c < NQ; c++) acc[sg][par][c] = mfma64(aP, bb[c * 512], acc[sg][par][c]);
+            }
           }
         }
       }
+    }
+    if (more && active) {
+#pragma unroll
+      for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+        for (int par = 0; par < 2; par++) {
+          pc[ks][par] = pn[ks][par];
+          dpc[ks][par] = dpn[ks][par];
+        }
     }
   }
   if (!active) return;
@@ -153,7 +194,7 @@ __global__ void __launch_bounds__(256) k_leg_bwd(
     if (sg >= nsig) continue;
     const bool self = (nsig == 1);
     const double mc = (sg == 0) ? -(double)M : (double)M;  // m of the conjugate slot
-    const long long base = ga.slot_off[slot[sg]] * ga.slot_mul + (long long)k * ga.s_k;
+    const long long base = ga.slot_off[sg == 0 ? L + M : L - M] * ga.slot_mul + (long long)k * ga.s_k;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int t = t0 + 4 * r + (lane >> 4);
@@ -161,7 +202,7 @@ __global__ void __launch_bounds__(256) k_leg_bwd(
       const bool eq = odd_nth && t == d.nth / 2;
       const long long on = base + (long long)t * ga.s_t;
       const long long os = base + (long long)(d.nth - 1 - t) * ga.s_t;
-      if (is_vec) {
+      if (VEC) {
         const double c1 = -asin_t[t] * mc * a1;
         const double E0 = acc[sg][0][0][r], O0 = acc[sg][1][0][r];
         const double E3 = acc[sg][0][3][r], O3 = acc[sg][1][3][r];
@@ -186,7 +227,7 @@ __global__ void __launch_bounds__(256) k_leg_bwd(
         }
       } else {
 #pragma unroll
-        for (int c = 0; c < kSclPer; c++) {
+        for (int c = 0; c < NQ; c++) {
           if (c >= nc) continue;
           const double E = acc[sg][0][c][r], O = acc[sg][1][c][r];
           double *o = out + (long long)(nd0 + c) * ga.s_nd;
@@ -207,106 +248,149 @@ __global__ void __launch_bounds__(256) k_leg_bwd(
 // component group, degree block); 4 waves, wave w owns pair indices
 // i in [64*ib + 16*w, +16) of both parities (l = M + 2i + parity).  The
 // reduction runs over hemisphere rows in chunks of 16 staged in LDS as
-// weighted symmetric / antisymmetric combinations of the mirrored rows.
+// weighted symmetric / antisymmetric combinations of the mirrored rows; the
+// rows of chunk tc+1 and the P/dP fragments are prefetched into registers
+// while chunk tc is reduced.
+template <bool VEC>
 __global__ void __launch_bounds__(256) k_leg_fwd(
-    LegDims d, const double *__restrict__ in, GridAddr ga, double *__restrict__ sp,
+    LegDims d, int cg0, const double *__restrict__ in, GridAddr ga, double *__restrict__ sp,
     SpecAddr sa, const int *__restrict__ jst, const long long *__restrict__ offf,
     const int *__restrict__ ipad, const double *__restrict__ Pf,
     const double *__restrict__ dPf, const double *__restrict__ radius,
     const double *__restrict__ weight, const double *__restrict__ asin_t) {
-  __shared__ double sA[2 * 4 * 2 * 16 * 16];  // [sig][f][sym/asym][tt][kk]
+  constexpr int NC = VEC ? 3 : kSclPer;
+  constexpr int NQ = VEC ? 5 : kSclPer;
+  constexpr int NU = NC * 2;  // staging items per thread: 2 sig x NC x 256 / 256
+  constexpr int SBUF = 2 * NC * 2 * 256;
+  __shared__ double sA[2][SBUF];  // [buf][sig][f][sym/asym][tt][kk]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   long long b = blockIdx.x;
   const int ib = (int)(b % d.nblk);
   b /= d.nblk;
-  const int cg = (int)(b % d.ncg);
+  const int cgl = (int)(b % d.ncg);
   b /= d.ncg;
   const int rt = (int)(b % d.n_rt);
   const int M = (int)(b / d.n_rt);
   const int L = d.L;
-  const int ip[2] = {ipad[2 * M], ipad[2 * M + 1]};
-  if (ib * 64 >= ip[0]) return;  // whole work-group beyond the degrees of M
+  const int ip0 = ipad[2 * M], ip1 = ipad[2 * M + 1];
+  if (ib * 64 >= ip0) return;  // whole work-group beyond the degrees of M
   const int nsig = (M == 0) ? 1 : 2;
   const int k0 = rt * 16;
   const int i0 = ib * 64 + wave * 16;
-  const bool active = i0 < ip[0];
-  const bool is_vec = cg < d.nv;
-  const int nc = is_vec ? 3 : min(kSclPer, d.ns - kSclPer * (cg - d.nv));
-  const int nd0 = is_vec ? 3 * cg : 3 * d.nv + kSclPer * (cg - d.nv);
-  const long long of[2] = {offf[2 * M], offf[2 * M + 1]};
-  const int slot[2] = {L + M, L - M};
+  const bool active = i0 < ip0;
+  const int cg = cg0 + cgl;
+  const int nc = VEC ? 3 : min(kSclPer, d.ns - kSclPer * (cg - d.nv));
+  const int nd0 = VEC ? 3 * cg : 3 * d.nv + kSclPer * (cg - d.nv);
+  const long long of0 = offf[2 * M], of1 = offf[2 * M + 1];
+  const int slot0 = L + M, slot1 = L - M;
   const bool odd_nth = (d.nth & 1) != 0;
+  const int t_eq = odd_nth ? d.nth / 2 : -1;
 
-  d4 acc[2][2][kVecQ];
+  d4 acc[2][2][NQ];
 #pragma unroll
   for (int s = 0; s < 2; s++)
 #pragma unroll
     for (int p = 0; p < 2; p++)
 #pragma unroll
-      for (int q = 0; q < kVecQ; q++) acc[s][p][q] = d4{0.0, 0.0, 0.0, 0.0};
+      for (int q = 0; q < NQ; q++) acc[s][p][q] = d4{0.0, 0.0, 0.0, 0.0};
 
-  const int nchunk = (d.nth_h + 15) / 16;
-  for (int tc = 0; tc < nchunk; tc++) {
-    __syncthreads();
-    for (int idx = tid; idx < 2 * 4 * 16 * 16; idx += 256) {
-      const int kk = idx & 15, tt = (idx >> 4) & 15, f = (idx >> 8) & 3, sg = idx >> 10;
+  double vn[NU], vs[NU];
+  auto stage_load = [&](int tc) {
+#pragma unroll
+    for (int u = 0; u < NU; u++) {
+      const int idx = tid + 256 * u;
+      const int kk = idx & 15, tt = (idx >> 4) & 15, f = (idx >> 8) % NC, sg = idx / (256 * NC);
       const int t = tc * 16 + tt, k = k0 + kk;
       const bool valid = sg < nsig && f < nc && t < d.nth_h && k < d.nr;
-      double sym = 0.0, asym = 0.0;
-      if (valid) {
-        const int sl = sg == 0 ? slot[0] : slot[1];
-        const long long base = ga.slot_off[sl] * ga.slot_mul +
-                               (long long)(nd0 + f) * ga.s_nd + (long long)k * ga.s_k;
-        const double w = weight[t];
-        const double vn = in[base + (long long)t * ga.s_t];
-        if (odd_nth && t == d.nth / 2) {
-          sym = vn * w;
-        } else {
-          const double vs = in[base + (long long)(d.nth - 1 - t) * ga.s_t];
-          sym = (vn + vs) * w;
-          asym = (vn - vs) * w;
-        }
-      }
-      double *dst = sA + (((sg * 4 + f) * 2) * 16 + tt) * 16 + kk;
-      dst[0] = sym;
-      dst[256] = asym;
+      const long long base = ga.slot_off[sg == 0 ? slot0 : slot1] * ga.slot_mul +
+                             (long long)(nd0 + f) * ga.s_nd + (long long)k * ga.s_k;
+      vn[u] = valid ? in[base + (long long)t * ga.s_t] : 0.0;
+      vs[u] = (valid && t != t_eq) ? in[base + (long long)(d.nth - 1 - t) * ga.s_t] : 0.0;
     }
-    __syncthreads();
-    if (!active) continue;
+  };
+  auto stage_store = [&](int tc, double *buf) {
+#pragma unroll
+    for (int u = 0; u < NU; u++) {
+      const int idx = tid + 256 * u;
+      const int kk = idx & 15, tt = (idx >> 4) & 15, f = (idx >> 8) % NC, sg = idx / (256 * NC);
+      const int t = tc * 16 + tt;
+      const double w = t < d.nth_h ? weight[t] : 0.0;
+      double *dst = buf + (((sg * NC + f) * 2) * 16 + tt) * 16 + kk;
+      dst[0] = (vn[u] + vs[u]) * w;
+      dst[256] = (t == t_eq) ? 0.0 : (vn[u] - vs[u]) * w;
+    }
+  };
+  double pc[4][2], dpc[4][2], pn[4][2], dpn[4][2], asc[4], asn_[4];
+  auto pload = [&](int tc, double (&a)[4][2], double (&da)[4][2], double (&as)[4]) {
 #pragma unroll
     for (int ks = 0; ks < 4; ks++) {
-      const int tt = ks * 4 + (lane >> 4);
-      const int t = tc * 16 + tt;
-      const double asn = (t < d.nth_h) ? asin_t[t] : 0.0;
+      const int t = tc * 16 + ks * 4 + (lane >> 4);
+      const long long c = i0 + (lane & 15);
+      const bool v1 = i0 < ip1;
+      a[ks][0] = Pf[of0 + (long long)t * ip0 + c];
+      a[ks][1] = v1 ? Pf[of1 + (long long)t * ip1 + c] : 0.0;
+      da[ks][0] = VEC ? dPf[of0 + (long long)t * ip0 + c] : 0.0;
+      da[ks][1] = (VEC && v1) ? dPf[of1 + (long long)t * ip1 + c] : 0.0;
+      as[ks] = (VEC && t < d.nth_h) ? asin_t[t] : 0.0;
+    }
+  };
+
+  const int nchunk = (d.nth_h + 15) / 16;
+  stage_load(0);
+  if (active) pload(0, pc, dpc, asc);
+  for (int tc = 0; tc < nchunk; tc++) {
+    double *buf = sA[tc & 1];
+    stage_store(tc, buf);
+    __syncthreads();
+    const bool more = tc + 1 < nchunk;
+    if (more) {
+      stage_load(tc + 1);
+      if (active) pload(tc + 1, pn, dpn, asn_);
+    }
+    if (active) {
 #pragma unroll
-      for (int par = 0; par < 2; par++) {
-        if (i0 >= ip[par]) continue;
-        const long long bo = of[par] + (long long)t * ip[par] + i0 + (lane & 15);
-        const double bP = Pf[bo];
-        const double bdP = is_vec ? dPf[bo] : 0.0;
-        const int sP = par, sdP = 1 - par;  // P: sym for even, dP: asym for even
+      for (int ks = 0; ks < 4; ks++) {
+        const int tt = ks * 4 + (lane >> 4);
+        const double asn = asc[ks];
 #pragma unroll
-        for (int sg = 0; sg < 2; sg++) {
-          if (sg >= nsig) continue;
-          const int sc = (nsig == 1) ? 0 : 1 - sg;
-          const double *aa = sA + ((sg * 4 * 2) * 16 + tt) * 16 + (lane & 15);
-          const double *ac = sA + ((sc * 4 * 2) * 16 + tt) * 16 + (lane & 15);
-          if (is_vec) {
-            const double avr = aa[(0 * 2 + sP) * 256];
-            const double avt = aa[(1 * 2 + sdP) * 256];
-            const double avp = aa[(2 * 2 + sdP) * 256];
-            const double avpc = ac[(2 * 2 + sP) * 256] * asn;
-            const double avtc = ac[(1 * 2 + sP) * 256] * asn;
-            acc[sg][par][0] = mfma64(avr, bP, acc[sg][par][0]);
-            acc[sg][par][1] = mfma64(avt, bdP, acc[sg][par][1]);
-            acc[sg][par][2] = mfma64(avpc, bP, acc[sg][par][2]);
-            acc[sg][par][3] = mfma64(avp, bdP, acc[sg][par][3]);
-            acc[sg][par][4] = mfma64(avtc, bP, acc[sg][par][4]);
-          } else {
+        for (int par = 0; par < 2; par++) {
+          if (i0 >= (par ? ip1 : ip0)) continue;
+          const double bP = pc[ks][par], bdP = dpc[ks][par];
+          const int sP = par, sdP = 1 - par;  // P: sym for even, dP: asym for even
 #pragma unroll
-            for (int c = 0; c < kSclPer; c++)
-              acc[sg][par][c] = mfma64(aa[(c * 2 + sP) * 256], bP, acc[sg][par][c]);
+          for (int sg = 0; sg < 2; sg++) {
+            if (sg >= nsig) continue;
+            const int sc = (nsig == 1) ? 0 : 1 - sg;
+            const double *aa = buf + ((sg * NC * 2) * 16 + tt) * 16 + (lane & 15);
+            const double *ac = buf + ((sc * NC * 2) * 16 + tt) * 16 + (lane & 15);
+            if (VEC) {
+              const double avr = aa[(0 * 2 + sP) * 256];
+              const double avt = aa[(1 * 2 + sdP) * 256];
+              const double avp = aa[(2 * 2 + sdP) * 256];
+              const double avpc = ac[(2 * 2 + sP) * 256] * asn;
+              const double avtc = ac[(1 * 2 + sP) * 256] * asn;
+              acc[sg][par][0] = mfma64(avr, bP, acc[sg][par][0]);
+              acc[sg][par][1] = mfma64(avt, bdP, acc[sg][par][1]);
+              acc[sg][par][2] = mfma64(avpc, bP, acc[sg][par][2]);
+              acc[sg][par][3] = mfma64(avp, bdP, acc[sg][par][3]);
+              acc[sg][par][4] = mfma64(avtc, bP, acc[sg][par][4]);
+            } else {
+#pragma unroll
+              for (int c = 0; c < NQ; c++)
+                acc[sg][par][c] = mfma64(aa[(c * 2 + sP) * 256], bP, acc[sg][par][c]);
+            }
           }
+        }
+      }
+    }
+    if (more && active) {
+#pragma unroll
+      for (int ks = 0; ks < 4; ks++) {
+        asc[ks] = asn_[ks];
+#pragma unroll
+        for (int par = 0; par < 2; par++) {
+          pc[ks][par] = pn[ks][par];
+          dpc[ks][par] = dpn[ks][par];
         }
       }
     }
@@ -327,15 +411,15 @@ __global__ void __launch_bounds__(256) k_leg_fwd(
       for (int sg = 0; sg < 2; sg++) {
         if (sg >= nsig) continue;
         const double ms = (sg == 0) ? (double)M : -(double)M;
-        const long long o = (long long)(jst[slot[sg]] + l - M) * sa.s_j + (long long)k * sa.s_k +
-                            (long long)nd0 * sa.s_nd;
-        if (is_vec) {
+        const long long o = (long long)(jst[sg == 0 ? slot0 : slot1] + l - M) * sa.s_j +
+                            (long long)k * sa.s_k + (long long)nd0 * sa.s_nd;
+        if (VEC) {
           sp[o] = r2 * g7 * acc[sg][par][0][r];
           sp[o + sa.s_nd] = r1 * g7 * (acc[sg][par][1][r] - ms * acc[sg][par][2][r]);
           sp[o + 2 * sa.s_nd] = r1 * g7 * (-acc[sg][par][3][r] - ms * acc[sg][par][4][r]);
         } else {
 #pragma unroll
-          for (int c = 0; c < kSclPer; c++)
+          for (int c = 0; c < NQ; c++)
             if (c < nc) sp[o + (long long)c * sa.s_nd] = g6 * acc[sg][par][c][r];
         }
       }
@@ -360,26 +444,48 @@ static LegDims make_dims(const sht_plan *p, int nv, int ns, int nblk) {
 
 int launch_leg_bwd(sht_plan *p, int nv, int ns, const double *sp, SpecAddr sa, double *out,
                    GridAddr ga) {
-  const LegDims d = make_dims(p, nv, ns, p->nth_hp / 64);
-  const long long nblocks = (long long)(p->L + 1) * d.n_rt * d.ncg * d.nblk;
-  if (nblocks == 0) return SHT_OK;
-  hipLaunchKernelGGL(k_leg_bwd, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d, sp, sa,
-                     out, ga, p->t.jst, p->t.offb, p->t.ipad, p->t.Pb, p->t.dPb, p->t.a_r,
-                     p->t.asin_t);
-  KSHT_HIP(hipGetLastError());
+  const int nsg = (ns + kSclPer - 1) / kSclPer;
+  for (int vec = 1; vec >= 0; vec--) {
+    const int ncg = vec ? nv : nsg;
+    if (ncg == 0) continue;
+    LegDims d = make_dims(p, nv, ns, p->nth_hp / 64);
+    d.ncg = ncg;
+    const long long nblocks = (long long)(p->L + 1) * d.n_rt * d.ncg * d.nblk;
+    const int cg0 = vec ? 0 : nv;
+    if (vec)
+      hipLaunchKernelGGL(k_leg_bwd<true>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d, cg0,
+                         sp, sa, out, ga, p->t.jst, p->t.offb, p->t.ipad, p->t.Pb, p->t.dPb,
+                         p->t.a_r, p->t.asin_t);
+    else
+      hipLaunchKernelGGL(k_leg_bwd<false>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d,
+                         cg0, sp, sa, out, ga, p->t.jst, p->t.offb, p->t.ipad, p->t.Pb, p->t.dPb,
+                         p->t.a_r, p->t.asin_t);
+    KSHT_HIP(hipGetLastError());
+  }
   return SHT_OK;
 }
 
 int launch_leg_fwd(sht_plan *p, int nv, int ns, const double *in, GridAddr ga, double *sp,
                    SpecAddr sa) {
   const int ne_max = (p->L + 2) / 2;
-  const LegDims d = make_dims(p, nv, ns, (ne_max + 63) / 64);
-  const long long nblocks = (long long)(p->L + 1) * d.n_rt * d.ncg * d.nblk;
-  if (nblocks == 0) return SHT_OK;
-  hipLaunchKernelGGL(k_leg_fwd, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d, in, ga,
-                     sp, sa, p->t.jst, p->t.offf, p->t.ipad, p->t.Pf, p->t.dPf, p->t.radius,
-                     p->t.weight, p->t.asin_t);
-  KSHT_HIP(hipGetLastError());
+  const int nsg = (ns + kSclPer - 1) / kSclPer;
+  for (int vec = 1; vec >= 0; vec--) {
+    const int ncg = vec ? nv : nsg;
+    if (ncg == 0) continue;
+    LegDims d = make_dims(p, nv, ns, (ne_max + 63) / 64);
+    d.ncg = ncg;
+    const long long nblocks = (long long)(p->L + 1) * d.n_rt * d.ncg * d.nblk;
+    const int cg0 = vec ? 0 : nv;
+    if (vec)
+      hipLaunchKernelGGL(k_leg_fwd<true>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d, cg0,
+                         in, ga, sp, sa, p->t.jst, p->t.offf, p->t.ipad, p->t.Pf, p->t.dPf,
+                         p->t.radius, p->t.weight, p->t.asin_t);
+    else
+      hipLaunchKernelGGL(k_leg_fwd<false>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d,
+                         cg0, in, ga, sp, sa, p->t.jst, p->t.offf, p->t.ipad, p->t.Pf, p->t.dPf,
+                         p->t.radius, p->t.weight, p->t.asin_t);
+    KSHT_HIP(hipGetLastError());
+  }
   return SHT_OK;
 }
 
