@@ -58,22 +58,23 @@ def flops_per_contraction(c):
     return 2.0 * ((c["nth"] + 1) // 2) * (c["L"] + 1) ** 2 * c["nr"]
 
 
-def run_engine(cfg_d, precision, args, torch):
-    from kemorin_mhd_amd import SHT_PREC_FP64, SHT_PREC_SPLIT_INT8, ShellConfig, SphTransPlan
-    cfg = ShellConfig(cfg_d["L"], cfg_d["nr"], cfg_d["nth"], cfg_d["nphi"])
-    prec = SHT_PREC_SPLIT_INT8 if precision == "split" else SHT_PREC_FP64
-    stream = torch.cuda.Stream()
-    plan = SphTransPlan(cfg, max_ncomp=9, precision=prec, n_slices=args.slices)
-    plan.set_stream(stream.cuda_stream)
+def make_config(cfg_d):
+    from kemorin_mhd_amd import ShellConfig
+    return ShellConfig(cfg_d["L"], cfg_d["nr"], cfg_d["nth"], cfg_d["nphi"])
+
+
+def make_inputs(cfg, torch, slope=1.5, seed=20261202):
+    """Seeded spectra sigma(l, r) = (l+1)^-slope sin(pi (r-ri)/(ro-ri)) for the
+    backward (8 comps), and a band-limited grid for the forward (9 comps), made
+    by the FP64 engine's backward of such spectra."""
+    from kemorin_mhd_amd import SHT_PREC_FP64, SphTransPlan, rlm_l_m
     dev = "cuda"
     gen = torch.Generator(device=dev)
-    gen.manual_seed(20261202)
-    # synthetic seeded spectra: sigma(l, r) = (l+1)^-1.5 sin(pi (r-ri)/(ro-ri))
-    from kemorin_mhd_amd import rlm_l_m
+    gen.manual_seed(seed)
     l, _ = rlm_l_m(cfg.L)
     r = cfg.radius()
     prof = torch.tensor(np.sin(np.pi * (r - cfg.ri) / (cfg.ro - cfg.ri)), device=dev)
-    sig = torch.tensor((l + 1.0) ** -1.5, device=dev)
+    sig = torch.tensor((l + 1.0) ** -slope, device=dev)
 
     def spectra(ncomp):
         x = torch.randn((cfg.nr, cfg.jmax, ncomp), generator=gen, dtype=torch.float64, device=dev)
@@ -81,14 +82,28 @@ def run_engine(cfg_d, precision, args, torch):
         x *= prof[:, None, None]
         return x.reshape(-1)
 
+    sp_b = spectra(8)
+    grid_f = torch.empty(cfg.n_rtp(9), dtype=torch.float64, device=dev)
+    tmp = spectra(9)
+    plan = SphTransPlan(cfg, max_ncomp=9, precision=SHT_PREC_FP64)
+    plan.sph_backward_transforms(3, 0, tmp, grid_f)
+    torch.cuda.synchronize()
+    plan.close()
+    del tmp
+    return sp_b, grid_f
+
+
+def run_engine(cfg, precision, args, torch, sp_b, grid_f, keep=False):
+    from kemorin_mhd_amd import SHT_PREC_FP64, SHT_PREC_SPLIT_INT8, SphTransPlan
+    prec = SHT_PREC_SPLIT_INT8 if precision == "split" else SHT_PREC_FP64
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    plan = SphTransPlan(cfg, max_ncomp=9, precision=prec, n_slices=args.slices)
+    plan.set_stream(stream.cuda_stream)
+    dev = "cuda"
     with torch.cuda.stream(stream):
-        sp_b = spectra(8)
         grid_b = torch.empty(cfg.n_rtp(8), dtype=torch.float64, device=dev)
         sp_f = torch.empty(cfg.n_rj(9), dtype=torch.float64, device=dev)
-        grid_f = torch.empty(cfg.n_rtp(9), dtype=torch.float64, device=dev)
-        tmp = spectra(9)
-        plan.sph_backward_transforms(3, 0, tmp, grid_f)  # band-limited forward input
-        del tmp
     stream.synchronize()
 
     def step():
@@ -110,16 +125,58 @@ def run_engine(cfg_d, precision, args, torch):
     timers = plan.timer_read()
     plan.timer_enable(False)
     dev_bytes = plan.device_bytes()
-    # round-trip consistency of the measured data (band-limited forward input)
-    check = None
-    if precision == "split":
-        check = "see tests/test_gpu_split.py"
     plan.close()
-    del sp_b, grid_b, sp_f, grid_f
+    out = (grid_b, sp_f) if keep else None
+    if not keep:
+        del grid_b, sp_f
     torch.cuda.empty_cache()
     ms = (t1 - t0) * 1e3 / args.steps
     return dict(ms_per_step=ms, round_trips_per_s=1e3 / ms, timers=timers, plan_bytes=dev_bytes,
-                check=check)
+                outputs=out)
+
+
+FIELDS_BWD = (("vector1", 0, 3), ("vector2", 3, 6), ("scalar1", 6, 7), ("scalar2", 7, 8))
+FIELDS_FWD = (("vector1", 0, 3), ("vector2", 3, 6), ("vector3", 6, 9))
+
+
+def field_errors(cfg, torch, out, ref):
+    """Per-field relative L2 and max-norm errors of `out` against `ref` for the
+    backward grid (comp blocks of nnod) and the forward spectra (comp fastest)."""
+    nnod = cfg.nr * cfg.nth * cfg.nphi
+    gb, sf = out
+    rb, rf = ref
+    res = {"backward": {}, "forward": {}}
+    for name, c0, c1 in FIELDS_BWD:
+        x, y = gb[c0 * nnod:c1 * nnod], rb[c0 * nnod:c1 * nnod]
+        d = x - y
+        res["backward"][name] = {"rel_l2": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(y)),
+                                 "rel_max": float(d.abs().max() / y.abs().max())}
+    xs, ys = sf.view(-1, 9), rf.view(-1, 9)
+    for name, c0, c1 in FIELDS_FWD:
+        d = xs[:, c0:c1] - ys[:, c0:c1]
+        y = ys[:, c0:c1]
+        res["forward"][name] = {"rel_l2": float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(y)),
+                                "rel_max": float(d.abs().max() / y.abs().max())}
+    return res
+
+
+def split_accuracy(cfg, args, torch, results):
+    """Split engine vs the FP64-MFMA engine (itself within 1e-12 of the CPU
+    FP64 oracle at parity-test sizes) on the benchmark spectra and on the
+    steeper (l+1)^-3 spectra."""
+    acc = {"slope_1.5": field_errors(cfg, torch, results["split"]["outputs"],
+                                     results["fp64"]["outputs"])}
+    for p in results.values():
+        p["outputs"] = None
+    torch.cuda.empty_cache()
+    sp3, gf3 = make_inputs(cfg, torch, slope=3.0, seed=7)
+    one = argparse.Namespace(steps=1, warmup=0, slices=args.slices)
+    o64 = run_engine(cfg, "fp64", one, torch, sp3, gf3, keep=True)["outputs"]
+    osp = run_engine(cfg, "split", one, torch, sp3, gf3, keep=True)["outputs"]
+    acc["slope_3"] = field_errors(cfg, torch, osp, o64)
+    del o64, osp, sp3, gf3
+    torch.cuda.empty_cache()
+    return acc
 
 
 def kernel_roofline(cfg_d, res, precision, steps):
@@ -220,10 +277,18 @@ def main():
         precs.append("fp64")
     if split_ok and args.precision in ("auto", "both", "split"):
         precs.append("split")
+    cfg = make_config(cfg_d)
+    sp_b, grid_f = make_inputs(cfg, torch)
     results = {}
+    keep = "split" in precs and "fp64" in precs
     for p in precs:
-        results[p] = run_engine(cfg_d, p, args, torch)
+        results[p] = run_engine(cfg, p, args, torch, sp_b, grid_f, keep=keep)
         print(f"# {p}: {results[p]['ms_per_step']:.3f} ms/round trip", file=sys.stderr, flush=True)
+    accuracy = None
+    if keep:
+        del sp_b, grid_f
+        accuracy = split_accuracy(cfg, args, torch, results)
+        print(f"# split accuracy: {json.dumps(accuracy)}", file=sys.stderr, flush=True)
     head = "split" if "split" in results else "fp64"
     res = results[head]
     roof, per_stage = kernel_roofline(cfg_d, res, head, args.steps)
@@ -249,6 +314,7 @@ def main():
     }
     if "split" in results and "fp64" in results:
         out["speedup_split_vs_fp64"] = results["fp64"]["ms_per_step"] / results["split"]["ms_per_step"]
+        out["split_error_vs_fp64"] = accuracy
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg_d, args.cpu_radii)
     print(json.dumps(out))

@@ -1,18 +1,27 @@
 // sht_split.hip -- split-precision Legendre transforms on INT8 MFMA.
 //
 // Same contractions as sht_legendre.hip (per order M, both slots +-M, even /
-// odd l-M, fused pack and unpack), but every 64-wide chunk of the reduction
-// index runs as an Ozaki-style sliced product:
+// odd l-M, fused pack and unpack), with every 64-wide chunk of the reduction
+// index run as an Ozaki-style sliced product:
 //   A = 2^(eA(row)+2) sum_s a_s 2^-8(s+1),  B = 2^(eB(col)+2) sum_t b_t 2^-8(t+1),
-// a_s, b_t balanced base-256 digits in [-128, 127] (round to nearest), with
-// per-row exponents of the P tables per chunk (precomputed in FP64 at plan
-// time) and per-column exponents of the data operand per chunk (computed on
-// the fly).  Products with s + t = g are summed exactly in INT32 by
-// v_mfma_i32_16x16x64_i8 (|sum| <= S * 64 * 128^2 < 2^31) for the S groups
-// g < S (triangular truncation), and each group is folded into the FP64
-// accumulator as ldexp((double)acc_g, eA + eB + 4 - 8(g+2)) at the end of
-// the chunk.  FP64 remains on the GPU for P generation, Gauss weights, the
-// operand exponents and this recombination.
+// a_s, b_t balanced base-256 digits (round to nearest; a_0 in [-64, 64], the
+// others in [-128, 127]), with per-row exponents of the P tables per chunk
+// (sliced from the FP64 tables at plan time) and per-column exponents of the
+// data operand per chunk (sliced once per call by k_bslice / k_fslice).
+// Products with s + t = g are summed exactly in INT32 by
+// v_mfma_i32_16x16x64_i8 for the groups g < S (triangular truncation).  With
+// |a_0|, |b_0| <= 64 and the other digits <= 128, |acc_g| <= 2^20 g for g >= 1
+// (2^18 for g = 0), so two groups combine exactly in INT32 as
+// 256 acc_g + acc_(g+1) (< 2^28 * 6 + 2^23 < 2^31 for g <= 6), and each such
+// pair is folded into the FP64 accumulators as
+// ldexp((double)pair, eA + eB + 4 - 8(g+3)).  FP64 remains on the GPU for P
+// generation, Gauss weights, the operand exponents and this recombination.
+//
+// Slices of the data operand are stored in MFMA-fragment order: a 1 KiB block
+// holds 16 columns x 64 K bytes with lane L's 16 bytes (column L%16,
+// K = 16 (L/16) .. +15) at offset 16 L, so a wave moves a block with one
+// 16-byte load per lane, straight from global memory into LDS.  P-table
+// slices are row-major 64-byte rows (row r, K bytes 0..63).
 //
 // INT8 MFMA layout (sht_selftest): A(16x64) lane holds row lane%16, bytes
 // K = 16*(lane/16) .. +15; B(64x16) lane holds column lane%16, the same K
@@ -35,6 +44,14 @@ struct SplitState {
   int *beP = nullptr, *bedP = nullptr;                  // backward row exponents
   signed char *fP = nullptr, *fdP = nullptr, *fPa = nullptr;  // forward slices
   int *feP = nullptr, *fedP = nullptr, *fePa = nullptr;
+  // data-operand slices (one transform at a time per plan, so the backward
+  // and forward share them)
+  std::vector<long long> hcb;  // backward: first chunk of (slot, parity)
+  long long *cb = nullptr;
+  long long nchb = 0;          // backward chunks over all (slot, parity)
+  int nchf = 0;                // forward chunks of 64 hemisphere rows
+  signed char *dig = nullptr;  // digits, fragment order
+  int *dexp = nullptr;         // per (chunk, component, radius) exponents
 };
 
 __device__ __forceinline__ i4 mfma8(i4 a, i4 b, i4 c) {
@@ -184,32 +201,163 @@ __device__ __forceinline__ double g6s_of(int l, int M) {
   return M == 0 ? (double)(2 * l + 1) / 2.0 : (double)(2 * l + 1) / 4.0;
 }
 
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+// byte offset of (column c16, K index q) in a fragment-order block
+__device__ __forceinline__ int frag_off(int c16, int q) {
+  return (((q >> 4) << 4) + c16) * 16 + (q & 15);
+}
+
+template <int S>
+__device__ __forceinline__ void put_digits(signed char *blk, long long sstride, int off,
+                                           long long Fb) {
+#pragma unroll
+  for (int s = 0; s < S; s++) blk[s * sstride + off] = (signed char)digit_byte<S>(Fb, s);
+}
+
+constexpr int kSliceThreads = 256;
+
+// ---------------------------------------------------------------------------
+// Backward data operand.  Work-group = (radius k < nr_p, slot m + L); the
+// slot's spectra at radius k are one contiguous run of n * ncomp doubles.
+// Wave task = (chunk of 64 pair indices over both parities, component):
+// lane = pair index i of l = M + 2i + parity.  The wave takes the chunk's
+// exponent, writes S digit bytes per lane in fragment order and the exponent.
+// The first component of each vector (pol) is scaled by g3 = l(l+1) (1/2 at
+// l = 0) as in the FP64 kernels; radii nr <= k < nr_p get zero digits.
+template <int S>
+__global__ void __launch_bounds__(kSliceThreads) k_bslice(
+    int L, int nr, int nr_p, int nv, int ncomp, const double *__restrict__ sp, SpecAddr sa,
+    const int *__restrict__ jst, const long long *__restrict__ cb, signed char *__restrict__ dig,
+    int *__restrict__ dexp) {
+  const int k = blockIdx.x, sl = blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int M = sl >= L ? sl - L : L - sl;
+  const int n = L - M + 1, np0 = (n + 1) / 2, np1 = n / 2;
+  const int nch0 = (np0 + 63) / 64, nch1 = (np1 + 63) / 64;
+  const int ntile = nr_p >> 4;
+  const int off = frag_off(k & 15, lane);
+  const long long sstride = (long long)ntile * 1024;
+  const double *src = sp + (long long)jst[sl] * sa.s_j + (long long)k * sa.s_k;
+  for (int task = wave; task < (nch0 + nch1) * ncomp; task += kSliceThreads / 64) {
+    const int cq = task % ncomp, cc = task / ncomp;
+    const int par = cc >= nch0 ? 1 : 0, c = par ? cc - nch0 : cc;
+    const int i = c * 64 + lane;
+    double x = 0.0;
+    if (k < nr && i < (par ? np1 : np0)) {
+      x = src[(long long)(par + 2 * i) * sa.s_j + (long long)cq * sa.s_nd];
+      if (cq < 3 * nv && cq % 3 == 0) x *= g3s_of(M + 2 * i + par);
+    }
+    const int e = wave_max(exp_of(x));
+    const long long gb = (cb[2 * sl + par] + c) * ncomp + cq;
+    put_digits<S>(dig + (gb * S * ntile + (k >> 4)) * 1024, sstride, off, fix_of<S>(x, e));
+    if (lane == 0) dexp[gb * nr_p + k] = e;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward data operand.  Work-group = (radius tile of 16, slot m + L).  Per
+// (chunk of 64 hemisphere rows, component) the tile's rows are staged in LDS
+// (loads ordered along the smaller of the radius and colatitude strides) as
+// the weighted symmetric and antisymmetric combinations
+// (v(t) +- v(nth-1-t)) w(t), as in the FP64 kernel (an odd grid's equator row
+// is its own mirror: (v w, 0)); wave w then slices radii 4w .. 4w+3 with
+// lane = row.
+template <int S>
+__global__ void __launch_bounds__(kSliceThreads) k_fslice(
+    int nr, int nr_p, int nth, int nth_h, int nchf, int ncomp, const double *__restrict__ in,
+    GridAddr ga, const double *__restrict__ weight, signed char *__restrict__ dig,
+    int *__restrict__ dexp) {
+  __shared__ double sSym[64][17], sAsym[64][17];
+  const int kt = blockIdx.x, sl = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ntile = nr_p >> 4;
+  const int t_eq = (nth & 1) ? nth / 2 : -1;
+  const bool k_fast = ga.s_k <= ga.s_t;
+  const long long base = ga.slot_off[sl] * ga.slot_mul;
+  const long long sstride = (long long)ntile * 1024;
+  for (int c = 0; c < nchf; c++) {
+    for (int cq = 0; cq < ncomp; cq++) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int idx = tid + kSliceThreads * u;
+        const int kk = k_fast ? (idx & 15) : (idx >> 6);
+        const int tt = k_fast ? (idx >> 4) : (idx & 63);
+        const int t = c * 64 + tt, k = kt * 16 + kk;
+        double xs = 0.0, xa = 0.0;
+        if (k < nr && t < nth_h) {
+          const long long b = base + (long long)cq * ga.s_nd + (long long)k * ga.s_k;
+          const double vn = in[b + (long long)t * ga.s_t];
+          const double vs = t != t_eq ? in[b + (long long)(nth - 1 - t) * ga.s_t] : 0.0;
+          const double w = weight[t];
+          xs = (vn + vs) * w;
+          xa = t != t_eq ? (vn - vs) * w : 0.0;
+        }
+        sSym[tt][kk] = xs;
+        sAsym[tt][kk] = xa;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int kk = wave * 4 + j;
+        const double xs = sSym[lane][kk], xa = sAsym[lane][kk];
+        const int es = wave_max(exp_of(xs)), ea = wave_max(exp_of(xa));
+        const long long g0 = (((long long)sl * nchf + c) * 2) * ncomp + cq;  // symmetric
+        const long long g1 = g0 + ncomp;                                       // antisymmetric
+        const int off = frag_off(kk, lane);
+        put_digits<S>(dig + (g0 * S * ntile + kt) * 1024, sstride, off, fix_of<S>(xs, es));
+        put_digits<S>(dig + (g1 * S * ntile + kt) * 1024, sstride, off, fix_of<S>(xa, ea));
+        if (lane == 0) {
+          dexp[g0 * nr_p + kt * 16 + kk] = es;
+          dexp[g1 * nr_p + kt * 16 + kk] = ea;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// GEMM side.  Work-groups of 8 waves; the data-operand blocks of a step are
+// copied global -> LDS (one 16-byte load per lane per 1 KiB block, LDS
+// address = block base + 16 lane) into one of two buffers while the other is
+// reduced.
+constexpr int kGemmThreads = 512;
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void gbl_void_t;
+
+__device__ __forceinline__ void copy_block(const signed char *src_lane, i4 *dst_block) {
+  __builtin_amdgcn_global_load_lds((gbl_void_t *)src_lane, (lds_void_t *)dst_block, 16, 0, 0);
+}
+__device__ __forceinline__ void wait_copies() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 struct SplitDims {
-  int L, nr, nth, nth_h, nth_hp, nv, ns, n_rt, ncg, nblk;
+  int L, nr, nr_p, nth, nth_h, nth_hp, ncomp, n_rt, ncg, nblk, nchf;
 };
-constexpr int kSplitScl = 4;
 
 // ---------------------------------------------------------------------------
 // Backward.  Work-group = (M, radius tile, component group, theta block of
-// 32 rows), 4 waves: wave w owns rows [16*(2 tb + w%2), +16) and parity
-// w/2 of l-M.  Per chunk of 64 pair indices i: a first pass over the
-// spectra takes per-column exponents (LDS atomicMax), a second pass slices
-// them into LDS, then S groups of INT8 MFMAs.  The odd-parity waves hand
-// their FP64 sums to the even-parity waves through LDS for the N/S
-// recombination.
-constexpr int kSplitThreads = 256;  // 4 waves: 2 tiles x 2 parities
-
-template <int S, bool VEC>
-__global__ void __launch_bounds__(kSplitThreads) k_leg_bwd_split(
-    SplitDims d, int cg0, const double *__restrict__ sp, SpecAddr sa, double *__restrict__ out,
-    GridAddr ga, const int *__restrict__ jst, SplitTabs tb, const int *__restrict__ ipad,
-    const double *__restrict__ a_r, const double *__restrict__ asin_t) {
-  constexpr int NC = VEC ? 3 : kSplitScl;
-  constexpr int NQ = VEC ? 5 : kSplitScl;
-  constexpr int SB_I4 = 2 * NC * 2 * S * 16 * 4;  // [sig][c][par][s][k][64 bytes]
-  constexpr int EX_I4 = (2 * 2 * NQ * 4 * 64 * 8 + 15) / 16;  // epilogue exchange (doubles)
-  __shared__ i4 sB[SB_I4 > EX_I4 ? SB_I4 : EX_I4];
-  __shared__ int sE[2 * NC * 2 * 16];
+// 64 rows); wave w owns theta tile w%4 for output slot w/4 (slot 0: m = +M,
+// slot 1: m = -M), over the steps (chunk c, parity) of both parities.  The
+// FP64 accumulators hold the north / south rows directly:
+//   even l-M: P terms add to both, dP terms (antisymmetric) to N and -S;
+//   odd l-M:  P terms N and -S, dP terms both,
+// and per output slot: vr = a^2 [P pol g3], vtheta = a ([dP dpol] - asin m' [P tor']),
+// vphi = a (-[dP tor] - asin m' [P dpol']), with primes on the conjugate slot.
+template <int S, int NB, bool VEC>
+__global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
+    SplitDims d, int nd_first, SplitTabs tb, const long long *__restrict__ cb,
+    const signed char *__restrict__ dig, const int *__restrict__ dexp, double *__restrict__ out,
+    GridAddr ga, const double *__restrict__ a_r, const double *__restrict__ asin_t) {
+  extern __shared__ i4 smem[];  // [2 buffers][2 slots][NB][S][64 lanes]
+  constexpr int NBLK = 2 * NB * S;
+  constexpr int NQ = VEC ? 5 : NB;
+  constexpr int NO = VEC ? 3 : NB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   long long bidx = blockIdx.x;
   const int tbk = (int)(bidx % d.nblk);
@@ -219,235 +367,206 @@ __global__ void __launch_bounds__(kSplitThreads) k_leg_bwd_split(
   const int rt = (int)(bidx % d.n_rt);
   const int M = (int)(bidx / d.n_rt);
   const int L = d.L;
-  const int n = L - M + 1;
-  const int np0 = (n + 1) / 2, np1 = n / 2;
-  const int nsig = (M == 0) ? 1 : 2;
-  const int k0 = rt * 16;
-  const int wt = wave & 1, par = wave >> 1;
-  const int t0 = (tbk * 2 + wt) * 16;
-  const bool active = t0 < d.nth_h;
-  const int npar = par ? np1 : np0;
-  const int cg = cg0 + cgl;
-  const int nc = VEC ? 3 : min(kSplitScl, d.ns - kSplitScl * (cg - d.nv));
-  const int nd0 = VEC ? 3 * cg : 3 * d.nv + kSplitScl * (cg - d.nv);
-  const int jb0 = jst[L + M], jb1 = jst[L - M];
-  // staging role: (sg, par, kk, iq): 16 pair indices per thread
-  const int iq = tid & 3, kk_s = (tid >> 2) & 15, par_s = (tid >> 6) & 1, sg_s = tid >> 7;
-  const int npar_s = par_s ? np1 : np0;
-  const int k_s = k0 + kk_s;
-  const double *src_s = sp + (long long)((sg_s == 0 ? jb0 : jb1) + par_s) * sa.s_j +
-                        (long long)k_s * sa.s_k + (long long)nd0 * sa.s_nd;
-  const bool sv = sg_s < nsig && k_s < d.nr;
+  const int n = L - M + 1, np0 = (n + 1) / 2, np1 = n / 2;
+  const int nch0 = (np0 + 63) / 64, nch1 = (np1 + 63) / 64;
+  const int nsig = M == 0 ? 1 : 2;
+  const int tt = wave & 3, sg = wave >> 2;
+  const int t0 = (tbk * 4 + tt) * 16;
+  const bool active = t0 < d.nth_h && sg < nsig;
+  const int nd0 = nd_first + NB * cgl;
+  const int ntile = d.nr_p >> 4;
+  const int sgc = nsig == 1 ? 0 : 1 - sg;  // conjugate slot
+  const int sl_own = sg == 0 ? L + M : L - M;
+  const int sl_oth = sgc == 0 ? L + M : L - M;
+  const int kcol = rt * 16 + (lane & 15);
 
-  double C[2][NQ][4];
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int q = 0; q < NQ; q++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) C[a][q][r] = 0.0;
+  auto issue = [&](int st) {
+    const int c = st >> 1, par = st & 1;
+    i4 *buf = smem + (st & 1) * NBLK * 64;
+    for (int b = wave; b < NBLK; b += kGemmThreads / 64) {
+      const int sgb = b / (NB * S), f = (b / S) % NB, s = b % S;
+      const int sl = sgb == 0 ? L + M : L - M;
+      const long long gb = (cb[2 * sl + par] + c) * d.ncomp + nd0 + f;
+      copy_block(dig + ((gb * S + s) * ntile + rt) * 1024 + lane * 16, buf + b * 64);
+    }
+  };
+  auto step_ok = [&](int st) { return st < 2 * nch0 && !((st & 1) && (st >> 1) >= nch1); };
 
-  const int nchunk = (np0 + 63) / 64;
-  for (int c = 0; c < nchunk; c++) {
-    __syncthreads();
-    for (int idx = tid; idx < 2 * NC * 2 * 16; idx += kSplitThreads) sE[idx] = kEmpty;
-    __syncthreads();
-    {
-      int ecol[NC];
+  double C[NO][2][4];
 #pragma unroll
-      for (int cc = 0; cc < NC; cc++) ecol[cc] = kEmpty;
+  for (int o = 0; o < NO; o++)
 #pragma unroll
-      for (int u = 0; u < 16; u++) {
-        const int i = c * 64 + iq * 16 + u;
-        const bool valid = sv && i < npar_s;
-        const double *src = src_s + (long long)(2 * i) * sa.s_j;
+    for (int h = 0; h < 2; h++)
 #pragma unroll
-        for (int cc = 0; cc < NC; cc++) {
-          double x = (valid && cc < nc) ? src[(long long)cc * sa.s_nd] : 0.0;
-          if (VEC && cc == 0) x *= g3s_of(M + 2 * i + par_s);
-          ecol[cc] = max(ecol[cc], exp_of(x));
-        }
-      }
-#pragma unroll
-      for (int cc = 0; cc < NC; cc++)
-        atomicMax(&sE[((sg_s * NC + cc) * 2 + par_s) * 16 + kk_s], ecol[cc]);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int cc = 0; cc < NC; cc++) {
-      const int e = sE[((sg_s * NC + cc) * 2 + par_s) * 16 + kk_s];
-      long long F[16];
-#pragma unroll
-      for (int u = 0; u < 16; u++) {
-        const int i = c * 64 + iq * 16 + u;
-        const bool valid = sv && i < npar_s && cc < nc;
-        double x = valid ? src_s[(long long)(2 * i) * sa.s_j + (long long)cc * sa.s_nd] : 0.0;
-        if (VEC && cc == 0) x *= g3s_of(M + 2 * i + par_s);
-        F[u] = fix_of<S>(x, e);
-      }
-#pragma unroll
-      for (int s = 0; s < S; s++)
-        sB[((((sg_s * NC + cc) * 2 + par_s) * S + s) * 16 + kk_s) * 4 + iq] =
-            i4{(int)pack4<S>(F, s), (int)pack4<S>(F + 4, s), (int)pack4<S>(F + 8, s),
-               (int)pack4<S>(F + 12, s)};
-    }
-    __syncthreads();
-    if (!active || c * 64 >= npar) continue;
-    const int row0 = t0 + 4 * (lane >> 4);
-    const long long so = tb.off[2 * M + par];
-    const long long eo = tb.eoff[2 * M + par] + (long long)c * d.nth_hp;
-    i4 aP[S], adP[S];
-#pragma unroll
-    for (int s = 0; s < S; s++) {
-      const long long ab = so + (((long long)c * S + s) * d.nth_hp + t0 + (lane & 15)) * 64 +
-                           16 * (lane >> 4);
-      aP[s] = *reinterpret_cast<const i4 *>(tb.P + ab);
-      if (VEC) adP[s] = *reinterpret_cast<const i4 *>(tb.dP + ab);
-    }
-    int eAP[4], eAD[4];
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      eAP[r] = tb.eP[eo + row0 + r];
-      eAD[r] = VEC ? tb.edP[eo + row0 + r] : kEmpty;
-    }
-    int eBc[2][NC];
-#pragma unroll
-    for (int sg = 0; sg < 2; sg++)
-#pragma unroll
-      for (int cc = 0; cc < NC; cc++) eBc[sg][cc] = sE[((sg * NC + cc) * 2 + par) * 16 + (lane & 15)];
-#pragma unroll
-    for (int g = 0; g < S; g++) {
-      i4 acc[2][NQ];
-#pragma unroll
-      for (int sg = 0; sg < 2; sg++)
-#pragma unroll
-        for (int q = 0; q < NQ; q++) acc[sg][q] = i4{0, 0, 0, 0};
-#pragma unroll
-      for (int s = 0; s <= g; s++) {
-        const int t = g - s;
-#pragma unroll
-        for (int sg = 0; sg < 2; sg++) {
-          if (sg >= nsig) continue;
-          const i4 *bb = sB + ((((sg * NC) * 2 + par) * S + t) * 16 + (lane & 15)) * 4 + (lane >> 4);
-          if (VEC) {
-            const i4 bpol = bb[0], bdpol = bb[2 * S * 64], btor = bb[2 * 2 * S * 64];
-            acc[sg][0] = mfma8(aP[s], bpol, acc[sg][0]);
-            acc[sg][1] = mfma8(aP[s], btor, acc[sg][1]);
-            acc[sg][2] = mfma8(aP[s], bdpol, acc[sg][2]);
-            acc[sg][3] = mfma8(adP[s], bdpol, acc[sg][3]);
-            acc[sg][4] = mfma8(adP[s], btor, acc[sg][4]);
-          } else {
-#pragma unroll
-            for (int q = 0; q < NQ; q++) acc[sg][q] = mfma8(aP[s], bb[q * 2 * S * 64], acc[sg][q]);
-          }
-        }
-      }
-      const int sh = 4 - 8 * (g + 2);  // 2^(eA+2) 2^(eB+2) 2^-8(g+2)
-#pragma unroll
-      for (int sg = 0; sg < 2; sg++) {
-        if (sg >= nsig) continue;
-#pragma unroll
-        for (int q = 0; q < NQ; q++) {
-          // component of quantity q: pol*g3, tor, dpol, dpol, tor
-          const int cq = VEC ? (q == 0 ? 0 : (q == 1 || q == 4) ? 2 : 1) : q;
-          const int eb = eBc[sg][cq];
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int ea = (VEC && q >= 3) ? eAD[r] : eAP[r];
-            C[sg][q][r] += ldexp((double)acc[sg][q][r], ea + eb + sh);
-          }
-        }
-      }
-    }
-  }
-  // odd-parity waves hand their sums to the even-parity waves
-  __syncthreads();
-  double *X = reinterpret_cast<double *>(sB);
-  if (par == 1) {
-#pragma unroll
-    for (int sg = 0; sg < 2; sg++)
-#pragma unroll
-      for (int q = 0; q < NQ; q++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) X[(((wt * 2 + sg) * NQ + q) * 4 + r) * 64 + lane] = C[sg][q][r];
-  }
-  __syncthreads();
-  if (par == 1 || !active) return;
-  const int k = k0 + (lane & 15);
-  if (k >= d.nr) return;
-  const double a1 = a_r[k], a2 = a1 * a1;
-  const bool odd_nth = (d.nth & 1) != 0;
-#define ODD(sg, q, r) X[(((wt * 2 + (sg)) * NQ + (q)) * 4 + (r)) * 64 + lane]
-#pragma unroll
-  for (int sg = 0; sg < 2; sg++) {
-    if (sg >= nsig) continue;
-    const bool self = (nsig == 1);
-    const int so_ = self ? sg : 1 - sg;
-    const double mc = (sg == 0) ? -(double)M : (double)M;
-    const long long base = ga.slot_off[sg == 0 ? L + M : L - M] * ga.slot_mul + (long long)k * ga.s_k;
+      for (int r = 0; r < 4; r++) C[o][h][r] = 0.0;
+  double fr[4];  // -asin(theta) m of the conjugate slot
+  {
+    const double mc = sg == 0 ? -(double)M : (double)M;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int t = t0 + 4 * (lane >> 4) + r;
-      if (t >= d.nth_h) continue;
-      const bool eq = odd_nth && t == d.nth / 2;
-      const long long on = base + (long long)t * ga.s_t;
-      const long long os = base + (long long)(d.nth - 1 - t) * ga.s_t;
-      if (VEC) {
-        const double c1 = -asin_t[t] * mc * a1;
-        const double E0 = C[sg][0][r], O0 = ODD(sg, 0, r);
-        const double E3 = C[sg][3][r], O3 = ODD(sg, 3, r);
-        const double E4 = C[sg][4][r], O4 = ODD(sg, 4, r);
-        const double Ec1 = self ? C[sg][1][r] : C[1 - sg][1][r], Oc1 = ODD(so_, 1, r);
-        const double Ec2 = self ? C[sg][2][r] : C[1 - sg][2][r], Oc2 = ODD(so_, 2, r);
-        double *o = out + (long long)nd0 * ga.s_nd;
-        if (eq) {
-          o[on] = a2 * E0;
-          o[on + ga.s_nd] = a1 * O3 + c1 * Ec1;
-          o[on + 2 * ga.s_nd] = -a1 * O4 + c1 * Ec2;
-        } else {
-          o[on] = a2 * (E0 + O0);
-          o[on + ga.s_nd] = a1 * (E3 + O3) + c1 * (Ec1 + Oc1);
-          o[on + 2 * ga.s_nd] = -a1 * (E4 + O4) + c1 * (Ec2 + Oc2);
-          o[os] = a2 * (E0 - O0);
-          o[os + ga.s_nd] = a1 * (O3 - E3) + c1 * (Ec1 - Oc1);
-          o[os + 2 * ga.s_nd] = -a1 * (O4 - E4) + c1 * (Ec2 - Oc2);
-        }
-      } else {
+      fr[r] = (VEC && t < d.nth_h) ? -asin_t[t] * mc : 0.0;
+    }
+  }
+
+  issue(0);
+  for (int st = 0; step_ok(st); st++) {
+    wait_copies();
+    __syncthreads();
+    if (step_ok(st + 1)) issue(st + 1);
+    if (!active) continue;
+    const int c = st >> 1, par = st & 1;
+    const i4 *buf = smem + (st & 1) * NBLK * 64;
+    // P / dP slices of this wave's theta tile
+    const long long ab = tb.off[2 * M + par] +
+                         ((long long)c * S * d.nth_hp + t0 + (lane & 15)) * 64 + 16 * (lane >> 4);
+    i4 aP[S], adP[S];
 #pragma unroll
-        for (int q = 0; q < NQ; q++) {
-          if (q >= nc) continue;
-          const double E = C[sg][q][r], O = ODD(sg, q, r);
-          double *o = out + (long long)(nd0 + q) * ga.s_nd;
-          if (eq) {
-            o[on] = E;
-          } else {
-            o[on] = E + O;
-            o[os] = E - O;
+    for (int s = 0; s < S; s++) {
+      aP[s] = *reinterpret_cast<const i4 *>(tb.P + ab + (long long)s * d.nth_hp * 64);
+      if (VEC) adP[s] = *reinterpret_cast<const i4 *>(tb.dP + ab + (long long)s * d.nth_hp * 64);
+    }
+    const long long eo = tb.eoff[2 * M + par] + (long long)c * d.nth_hp + t0 + 4 * (lane >> 4);
+    const long long go = (cb[2 * sl_own + par] + c) * d.ncomp + nd0;
+    const long long gc = (cb[2 * sl_oth + par] + c) * d.ncomp + nd0;
+    int ex[NQ][4];
+    if (VEC) {
+      const int ebp = dexp[(go + 0) * d.nr_p + kcol], ebd = dexp[(go + 1) * d.nr_p + kcol];
+      const int ebt = dexp[(go + 2) * d.nr_p + kcol];
+      const int ebtc = dexp[(gc + 2) * d.nr_p + kcol], ebdc = dexp[(gc + 1) * d.nr_p + kcol];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int eP = tb.eP[eo + r], eD = tb.edP[eo + r];
+        ex[0][r] = eP + ebp;
+        ex[1 % NQ][r] = eP + ebtc;
+        ex[2 % NQ][r] = eP + ebdc;
+        ex[3 % NQ][r] = eD + ebd;
+        ex[4 % NQ][r] = eD + ebt;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        const int eb = dexp[(go + q) * d.nr_p + kcol];
+#pragma unroll
+        for (int r = 0; r < 4; r++) ex[q][r] = tb.eP[eo + r] + eb;
+      }
+    }
+    const i4 *bo = buf + sg * NB * S * 64 + lane;
+    const i4 *bc = buf + sgc * NB * S * 64 + lane;
+    // quantity q: A table, data block, output, N / S coefficients
+#define AOF(q, s) ((VEC && (q) >= 3) ? adP[s] : aP[s])
+#define BOF(q, t)                                                            \
+  (VEC ? ((q) == 0   ? bo[(0 * S + (t)) * 64]                                \
+          : (q) == 1 ? bc[(2 * S + (t)) * 64]                                \
+          : (q) == 2 ? bc[(1 * S + (t)) * 64]                                \
+          : (q) == 3 ? bo[(1 * S + (t)) * 64]                                \
+                     : bo[(2 * S + (t)) * 64])                               \
+       : bo[((q) * S + (t)) * 64])
+    // quantities two at a time: 4 independent INT32 accumulator chains
+#pragma unroll
+    for (int qb = 0; qb < NQ; qb += 2) {
+      __builtin_amdgcn_sched_barrier(0);  // keep the register footprint to two quantities
+#pragma unroll
+      for (int g0 = 0; g0 < S; g0 += 2) {
+        const bool two = g0 + 1 < S;
+        i4 a0[2], a1[2];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          a0[j] = i4{0, 0, 0, 0};
+          a1[j] = i4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int t = 0; t < S; t++) {
+          if (t > g0 + 1) continue;
+          const int s0 = g0 - t, s1 = g0 + 1 - t;
+#pragma unroll
+          for (int j = 0; j < 2; j++) {
+            const int q = qb + j;
+            if (q >= NQ) continue;
+            const i4 b = BOF(q, t);
+            if (s0 >= 0) a0[j] = mfma8(AOF(q, s0), b, a0[j]);
+            if (two && s1 >= 0) a1[j] = mfma8(AOF(q, s1), b, a1[j]);
+          }
+        }
+        const int sh = two ? 4 - 8 * (g0 + 3) : 4 - 8 * (g0 + 2);
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          const int q = qb + j;
+          if (q >= NQ) continue;
+          const int o = VEC ? (q == 0 ? 0 : (q == 1 || q == 3) ? 1 : 2) : q;
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int v = two ? a0[j][r] * 256 + a1[j][r] : a0[j][r];
+            const double x = ldexp((double)v, ex[q][r] + sh);
+            double cn, cs;
+            if (!VEC || q == 0) {
+              cn = 1.0;
+              cs = par ? -1.0 : 1.0;
+            } else if (q <= 2) {
+              cn = fr[r];
+              cs = par ? -fr[r] : fr[r];
+            } else if (q == 3) {
+              cn = 1.0;
+              cs = par ? 1.0 : -1.0;
+            } else {
+              cn = -1.0;
+              cs = par ? -1.0 : 1.0;
+            }
+            C[o][0][r] = fma(x, cn, C[o][0][r]);
+            C[o][1][r] = fma(x, cs, C[o][1][r]);
           }
         }
       }
     }
+#undef AOF
+#undef BOF
   }
-#undef ODD
+  if (!active) return;
+  const int k = kcol;
+  if (k >= d.nr) return;
+  const double a1 = a_r[k], a2 = a1 * a1;
+  const bool odd_nth = (d.nth & 1) != 0;
+  const long long base = ga.slot_off[sl_own] * ga.slot_mul + (long long)k * ga.s_k;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int t = t0 + 4 * (lane >> 4) + r;
+    if (t >= d.nth_h) continue;
+    const bool eq = odd_nth && t == d.nth / 2;
+    const long long on = base + (long long)t * ga.s_t;
+    const long long os = base + (long long)(d.nth - 1 - t) * ga.s_t;
+#pragma unroll
+    for (int o = 0; o < NO; o++) {
+      const double sc = VEC ? (o == 0 ? a2 : a1) : 1.0;
+      double *dst = out + (long long)(nd0 + o) * ga.s_nd;
+      if (eq) {
+        dst[on] = sc * (0.5 * (C[o][0][r] + C[o][1][r]));
+      } else {
+        dst[on] = sc * C[o][0][r];
+        dst[os] = sc * C[o][1][r];
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
-// Forward.  Work-group = (M, radius tile, component group, degree block of
-// 32 pair indices), 4 waves: wave w owns rows i in [32 ib + 16 (w%2), +16)
-// of parity w/2.  Per chunk of 64 hemisphere rows: a first pass over the
-// grid coefficients takes the per-column exponents of the weighted
-// symmetric / antisymmetric combinations, a second pass slices them into
-// LDS, then S groups of INT8 MFMAs against the P, dP and P/sin(theta)
-// slices.
-template <int S, bool VEC>
-__global__ void __launch_bounds__(kSplitThreads) k_leg_fwd_split(
-    SplitDims d, int cg0, const double *__restrict__ in, GridAddr ga, double *__restrict__ sp,
-    SpecAddr sa, const int *__restrict__ jst, SplitTabs tb, const int *__restrict__ ipad,
-    const double *__restrict__ radius, const double *__restrict__ weight) {
-  constexpr int NC = VEC ? 3 : kSplitScl;
-  constexpr int NQ = VEC ? 5 : kSplitScl;
-  __shared__ i4 sB[2 * NC * 2 * S * 16 * 4];  // [sig][f][sym/asym][s][k][64 bytes]
-  __shared__ int sE[2 * NC * 2 * 16];
+// Forward.  Work-group = (M, radius tile, component group, block of 64 pair
+// indices); wave w owns rows i in [64 ib + 16 (w%4), +16) of parity w/4 for
+// both slots, over the chunks of 64 hemisphere rows.  Rows of even l-M take
+// the symmetric combinations against P and P/sin, the antisymmetric ones
+// against dP; odd l-M the other way round.  Per slot (m = +-M, primes on the
+// conjugate slot):
+//   pol = r^2 g7 [P vr], dpol = r g7 ([dP vt] - m [P/sin vp']),
+//   tor = r g7 (-[dP vp] - m [P/sin vt']); scalars g6 [P s].
+template <int S, int NB, bool VEC>
+__global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
+    SplitDims d, int nd_first, SplitTabs tb, const signed char *__restrict__ dig,
+    const int *__restrict__ dexp, double *__restrict__ sp, SpecAddr sa,
+    const int *__restrict__ jst, const int *__restrict__ ipad, const double *__restrict__ radius) {
+  extern __shared__ i4 smem[];  // [buffers][2 slots][sym/asym][NB][S][64 lanes]
+  constexpr int NBLK = 4 * NB * S;
+  constexpr int NBUF = (2 * NBLK <= 160) ? 2 : 1;
+  constexpr int NQ = VEC ? 5 : NB;
+  constexpr int NO = VEC ? 3 : NB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   long long bidx = blockIdx.x;
   const int ib = (int)(bidx % d.nblk);
@@ -458,181 +577,138 @@ __global__ void __launch_bounds__(kSplitThreads) k_leg_fwd_split(
   const int M = (int)(bidx / d.n_rt);
   const int L = d.L;
   const int ip0 = ipad[2 * M], ip1 = ipad[2 * M + 1];
-  if (ib * 32 >= ip0) return;
-  const int nsig = (M == 0) ? 1 : 2;
-  const int k0 = rt * 16;
-  const int par = wave >> 1;
-  const int i0 = ib * 32 + (wave & 1) * 16;
+  if (ib * 64 >= ip0) return;  // whole work-group beyond the degrees of M
+  const int nsig = M == 0 ? 1 : 2;
+  const int par = wave >> 2;
+  const int i0 = ib * 64 + (wave & 3) * 16;
   const int ip = par ? ip1 : ip0;
   const bool active = i0 < ip;
-  const int cg = cg0 + cgl;
-  const int nc = VEC ? 3 : min(kSplitScl, d.ns - kSplitScl * (cg - d.nv));
-  const int nd0 = VEC ? 3 * cg : 3 * d.nv + kSplitScl * (cg - d.nv);
-  const int slot0 = L + M, slot1 = L - M;
-  const bool odd_nth = (d.nth & 1) != 0;
-  const int t_eq = odd_nth ? d.nth / 2 : -1;
-  // staging role: (sg, kk, tq): 8 theta rows per thread
-  const int tq = tid & 7, kk_s = (tid >> 3) & 15, sg_s = tid >> 7;
-  const int k_s = k0 + kk_s;
-  const bool sv = sg_s < nsig && k_s < d.nr;
-  const long long base_s = ga.slot_off[sg_s == 0 ? slot0 : slot1] * ga.slot_mul +
-                           (long long)nd0 * ga.s_nd + (long long)k_s * ga.s_k;
+  const int nd0 = nd_first + NB * cgl;
+  const int ntile = d.nr_p >> 4;
+  const int kcol = rt * 16 + (lane & 15);
+  const int nchf = d.nchf;
 
-  double C[2][NQ][4];
+  auto issue = [&](int c) {
+    i4 *buf = smem + (c % NBUF) * NBLK * 64;
+    for (int b = wave; b < NBLK; b += kGemmThreads / 64) {
+      const int sgb = b / (2 * NB * S), h = (b / (NB * S)) & 1, f = (b / S) % NB, s = b % S;
+      const int sl = sgb == 0 ? L + M : L - M;
+      const long long gb = (((long long)sl * nchf + c) * 2 + h) * d.ncomp + nd0 + f;
+      copy_block(dig + ((gb * S + s) * ntile + rt) * 1024 + lane * 16, buf + b * 64);
+    }
+  };
+
+  double C[2][NO][4];
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
-    for (int q = 0; q < NQ; q++)
+    for (int o = 0; o < NO; o++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) C[a][q][r] = 0.0;
+      for (int r = 0; r < 4; r++) C[a][o][r] = 0.0;
 
-  const int nchunk = d.nth_hp / 64;
-  for (int c = 0; c < nchunk; c++) {
-    if (c * 64 >= d.nth_h) break;
+  const int hx = par, hy = 1 - par;  // P: symmetric for even l-M; dP the other one
+  issue(0);
+  for (int c = 0; c < nchf; c++) {
+    wait_copies();
     __syncthreads();
-    for (int idx = tid; idx < 2 * NC * 2 * 16; idx += kSplitThreads) sE[idx] = kEmpty;
-    __syncthreads();
-    {
-      int es[NC], ea[NC];
+    if (NBUF == 2 && c + 1 < nchf) issue(c + 1);
+    if (active) {
+      const i4 *buf = smem + (c % NBUF) * NBLK * 64;
+      const long long ab = tb.off[2 * M + par] + ((long long)c * S * ip + i0 + (lane & 15)) * 64 +
+                           16 * (lane >> 4);
+      auto load_a = [&](const signed char *tab, i4 (&a)[S]) {
 #pragma unroll
-      for (int f = 0; f < NC; f++) {
-        es[f] = kEmpty;
-        ea[f] = kEmpty;
+        for (int s = 0; s < S; s++)
+          a[s] = *reinterpret_cast<const i4 *>(tab + ab + (long long)s * ip * 64);
+      };
+      const long long eo = tb.eoff[2 * M + par] + (long long)c * ip + i0 + 4 * (lane >> 4);
+      int eP[4], eD[4], eA[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        eP[r] = tb.eP[eo + r];
+        eD[r] = VEC ? tb.edP[eo + r] : 0;
+        eA[r] = VEC ? tb.ePa[eo + r] : 0;
       }
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const int t = c * 64 + tq * 8 + u;
-        const bool valid = sv && t < d.nth_h;
-        const double w = valid ? weight[t] : 0.0;
-#pragma unroll
-        for (int f = 0; f < NC; f++) {
-          const bool vf = valid && f < nc;
-          const long long b = base_s + (long long)f * ga.s_nd;
-          const double xn = vf ? in[b + (long long)t * ga.s_t] : 0.0;
-          const double xs = (vf && t != t_eq) ? in[b + (long long)(d.nth - 1 - t) * ga.s_t] : 0.0;
-          es[f] = max(es[f], exp_of((xn + xs) * w));
-          ea[f] = max(ea[f], (t == t_eq) ? kEmpty : exp_of((xn - xs) * w));
-        }
-      }
-#pragma unroll
-      for (int f = 0; f < NC; f++) {
-        atomicMax(&sE[((sg_s * NC + f) * 2 + 0) * 16 + kk_s], es[f]);
-        atomicMax(&sE[((sg_s * NC + f) * 2 + 1) * 16 + kk_s], ea[f]);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int f = 0; f < NC; f++) {
-      const int e0 = sE[((sg_s * NC + f) * 2 + 0) * 16 + kk_s];
-      const int e1 = sE[((sg_s * NC + f) * 2 + 1) * 16 + kk_s];
-      long long Fs[8], Fa[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const int t = c * 64 + tq * 8 + u;
-        const bool vf = sv && t < d.nth_h && f < nc;
-        const double w = vf ? weight[t] : 0.0;
-        const long long b = base_s + (long long)f * ga.s_nd;
-        const double xn = vf ? in[b + (long long)t * ga.s_t] : 0.0;
-        const double xs = (vf && t != t_eq) ? in[b + (long long)(d.nth - 1 - t) * ga.s_t] : 0.0;
-        Fs[u] = fix_of<S>((xn + xs) * w, e0);
-        Fa[u] = fix_of<S>((t == t_eq) ? 0.0 : (xn - xs) * w, e1);
-      }
-#pragma unroll
-      for (int s = 0; s < S; s++) {
-        const unsigned ws0 = pack4<S>(Fs, s), ws1 = pack4<S>(Fs + 4, s);
-        const unsigned wa0 = pack4<S>(Fa, s), wa1 = pack4<S>(Fa + 4, s);
-        int *ds = reinterpret_cast<int *>(sB + ((((sg_s * NC + f) * 2 + 0) * S + s) * 16 + kk_s) * 4);
-        int *da = reinterpret_cast<int *>(sB + ((((sg_s * NC + f) * 2 + 1) * S + s) * 16 + kk_s) * 4);
-        ds[2 * tq] = (int)ws0;
-        ds[2 * tq + 1] = (int)ws1;
-        da[2 * tq] = (int)wa0;
-        da[2 * tq + 1] = (int)wa1;
-      }
-    }
-    __syncthreads();
-    if (!active) continue;
-    const int row0 = i0 + 4 * (lane >> 4);
-    const long long so = tb.off[2 * M + par];
-    const long long eo = tb.eoff[2 * M + par] + (long long)c * ip;
-    i4 aP[S], adP[S], aPa[S];
-#pragma unroll
-    for (int s = 0; s < S; s++) {
-      const long long ab = so + (((long long)c * S + s) * ip + i0 + (lane & 15)) * 64 + 16 * (lane >> 4);
-      aP[s] = *reinterpret_cast<const i4 *>(tb.P + ab);
+      const int sgc0 = nsig == 1 ? 0 : 1;  // conjugate of slot 0 (slot 1's is 0)
+      auto eb = [&](int sgb, int h, int f) {
+        const int sl = sgb == 0 ? L + M : L - M;
+        return dexp[((((long long)sl * nchf + c) * 2 + h) * d.ncomp + nd0 + f) * d.nr_p + kcol];
+      };
+      // quantity q of slot sg: data block (slot, sym/asym, component)
+#define QSLOT(q, sg) ((VEC && ((q) == 2 || (q) == 4)) ? ((sg) == 0 ? sgc0 : 0) : (sg))
+#define QH(q) ((!VEC || (q) == 0 || (q) == 2 || (q) == 4) ? hx : hy)
+#define QF(q) (!VEC ? (q) : (q) == 0 ? 0 : ((q) == 1 || (q) == 4) ? 1 : 2)
+#define BOF(q, sg, t) \
+  buf[(((QSLOT(q, sg) * 2 + QH(q)) * NB + QF(q)) * S + (t)) * 64 + lane]
+      // one quantity at a time for both slots (4 INT32 accumulator chains);
+      // AA = the A slices of its table, EA = their row exponents.  M = 0 has
+      // one slot: slot 1 then repeats slot 0 (same data) and is not stored.
+#define KSHT_FWD_QUANTITY(Q, AA, EA)                                                         \
+  {                                                                                         \
+    constexpr int q = (Q);                                                                  \
+    __builtin_amdgcn_sched_barrier(0);                                                      \
+    int exq[2][4];                                                                          \
+    _Pragma("unroll") for (int sg = 0; sg < 2; sg++) {                                      \
+      const int e = eb(QSLOT(q, sg), QH(q), QF(q));                                         \
+      _Pragma("unroll") for (int r = 0; r < 4; r++) exq[sg][r] = EA[r] + e;                 \
+    }                                                                                       \
+    constexpr int o = VEC ? (q == 0 ? 0 : (q <= 2) ? 1 : 2) : q;                            \
+    _Pragma("unroll") for (int g0 = 0; g0 < S; g0 += 2) {                                   \
+      const bool two = g0 + 1 < S;                                                          \
+      i4 a0[2], a1[2];                                                                      \
+      _Pragma("unroll") for (int j = 0; j < 2; j++) {                                       \
+        a0[j] = i4{0, 0, 0, 0};                                                             \
+        a1[j] = i4{0, 0, 0, 0};                                                             \
+      }                                                                                     \
+      _Pragma("unroll") for (int t = 0; t < S; t++) {                                       \
+        if (t > g0 + 1) continue;                                                           \
+        const int s0 = g0 - t, s1 = g0 + 1 - t;                                             \
+        _Pragma("unroll") for (int sg = 0; sg < 2; sg++) {                                  \
+          const i4 b = BOF(q, sg, t);                                                       \
+          if (s0 >= 0) a0[sg] = mfma8(AA[s0], b, a0[sg]);                                   \
+          if (two && s1 >= 0) a1[sg] = mfma8(AA[s1], b, a1[sg]);                            \
+        }                                                                                   \
+      }                                                                                     \
+      const int sh = two ? 4 - 8 * (g0 + 3) : 4 - 8 * (g0 + 2);                             \
+      _Pragma("unroll") for (int sg = 0; sg < 2; sg++) {                                    \
+        const double ms = sg == 0 ? (double)M : -(double)M;                                 \
+        const double cf = (!VEC || q <= 1) ? 1.0 : (q == 3 ? -1.0 : -ms);                   \
+        _Pragma("unroll") for (int r = 0; r < 4; r++) {                                     \
+          const int v = two ? a0[sg][r] * 256 + a1[sg][r] : a0[sg][r];                      \
+          C[sg][o][r] = fma(ldexp((double)v, exq[sg][r] + sh), cf, C[sg][o][r]);            \
+        }                                                                                   \
+      }                                                                                     \
+    }                                                                                       \
+  }
+      // table order P, dP, P/sin, each loaded just before its quantities
+      i4 aP[S], adP[S], aPa[S];
+      load_a(tb.P, aP);
       if (VEC) {
-        adP[s] = *reinterpret_cast<const i4 *>(tb.dP + ab);
-        aPa[s] = *reinterpret_cast<const i4 *>(tb.Pa + ab);
+        load_a(tb.dP, adP);
+        KSHT_FWD_QUANTITY(0, aP, eP)
+        load_a(tb.Pa, aPa);
+        KSHT_FWD_QUANTITY(1 % NQ, adP, eD)
+        KSHT_FWD_QUANTITY(3 % NQ, adP, eD)
+        KSHT_FWD_QUANTITY(2 % NQ, aPa, eA)
+        KSHT_FWD_QUANTITY(4 % NQ, aPa, eA)
+      } else {
+        KSHT_FWD_QUANTITY(0, aP, eP)
+        if (NQ > 1) KSHT_FWD_QUANTITY(1 % NQ, aP, eP)
+        if (NQ > 2) KSHT_FWD_QUANTITY(2 % NQ, aP, eP)
       }
+#undef KSHT_FWD_QUANTITY
+#undef QSLOT
+#undef QH
+#undef QF
+#undef BOF
     }
-    int eAP[4], eAD[4], eAA[4];
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      eAP[r] = tb.eP[eo + row0 + r];
-      eAD[r] = VEC ? tb.edP[eo + row0 + r] : kEmpty;
-      eAA[r] = VEC ? tb.ePa[eo + row0 + r] : kEmpty;
-    }
-    const int sP = par, sdP = 1 - par;  // P uses sym for even l-M, dP asym
-    constexpr int FS = 2 * S * 64;      // stride of f (in i4)
-    constexpr int HS = S * 64;          // stride of sym/asym
-#pragma unroll
-    for (int g = 0; g < S; g++) {
-      i4 acc[2][NQ];
-#pragma unroll
-      for (int sg = 0; sg < 2; sg++)
-#pragma unroll
-        for (int q = 0; q < NQ; q++) acc[sg][q] = i4{0, 0, 0, 0};
-#pragma unroll
-      for (int s = 0; s <= g; s++) {
-        const int t = g - s;
-#pragma unroll
-        for (int sg = 0; sg < 2; sg++) {
-          if (sg >= nsig) continue;
-          const int sc = (nsig == 1) ? 0 : 1 - sg;
-          const i4 *bb = sB + (((sg * NC * 2) * S + t) * 16 + (lane & 15)) * 4 + (lane >> 4);
-          const i4 *bc = sB + (((sc * NC * 2) * S + t) * 16 + (lane & 15)) * 4 + (lane >> 4);
-          if (VEC) {
-            const i4 bvr = bb[0 * FS + sP * HS], bvt = bb[1 * FS + sdP * HS], bvp = bb[2 * FS + sdP * HS];
-            const i4 bvpc = bc[2 * FS + sP * HS], bvtc = bc[1 * FS + sP * HS];
-            acc[sg][0] = mfma8(aP[s], bvr, acc[sg][0]);
-            acc[sg][1] = mfma8(adP[s], bvt, acc[sg][1]);
-            acc[sg][2] = mfma8(aPa[s], bvpc, acc[sg][2]);
-            acc[sg][3] = mfma8(adP[s], bvp, acc[sg][3]);
-            acc[sg][4] = mfma8(aPa[s], bvtc, acc[sg][4]);
-          } else {
-#pragma unroll
-            for (int q = 0; q < NQ; q++) acc[sg][q] = mfma8(aP[s], bb[q * FS + sP * HS], acc[sg][q]);
-          }
-        }
-      }
-      const int sh = 4 - 8 * (g + 2);  // 2^(eA+2) 2^(eB+2) 2^-8(g+2)
-#pragma unroll
-      for (int sg = 0; sg < 2; sg++) {
-        if (sg >= nsig) continue;
-        const int sc = (nsig == 1) ? 0 : 1 - sg;
-        const int lk = lane & 15;
-#pragma unroll
-        for (int q = 0; q < NQ; q++) {
-          int eb;
-          if (VEC) {
-            eb = q == 0 ? sE[((sg * NC + 0) * 2 + sP) * 16 + lk]
-               : q == 1 ? sE[((sg * NC + 1) * 2 + sdP) * 16 + lk]
-               : q == 2 ? sE[((sc * NC + 2) * 2 + sP) * 16 + lk]
-               : q == 3 ? sE[((sg * NC + 2) * 2 + sdP) * 16 + lk]
-                        : sE[((sc * NC + 1) * 2 + sP) * 16 + lk];
-          } else {
-            eb = sE[((sg * NC + q) * 2 + sP) * 16 + lk];
-          }
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int ea2 = VEC ? (q == 0 ? eAP[r] : (q == 1 || q == 3) ? eAD[r] : eAA[r]) : eAP[r];
-            C[sg][q][r] += ldexp((double)acc[sg][q][r], ea2 + eb + sh);
-          }
-        }
-      }
+    if (NBUF == 1 && c + 1 < nchf) {
+      __syncthreads();
+      issue(c + 1);
     }
   }
   if (!active) return;
-  const int k = k0 + (lane & 15);
+  const int k = kcol;
   if (k >= d.nr) return;
   const double r1 = radius[k], r2 = r1 * r1;
 #pragma unroll
@@ -644,23 +720,22 @@ __global__ void __launch_bounds__(kSplitThreads) k_leg_fwd_split(
 #pragma unroll
     for (int sg = 0; sg < 2; sg++) {
       if (sg >= nsig) continue;
-      const double ms = (sg == 0) ? (double)M : -(double)M;
-      const long long o = (long long)(jst[sg == 0 ? slot0 : slot1] + l - M) * sa.s_j +
+      const long long o = (long long)(jst[sg == 0 ? L + M : L - M] + l - M) * sa.s_j +
                           (long long)k * sa.s_k + (long long)nd0 * sa.s_nd;
       if (VEC) {
         sp[o] = r2 * g7 * C[sg][0][r];
-        sp[o + sa.s_nd] = r1 * g7 * (C[sg][1][r] - ms * C[sg][2][r]);
-        sp[o + 2 * sa.s_nd] = r1 * g7 * (-C[sg][3][r] - ms * C[sg][4][r]);
+        sp[o + sa.s_nd] = r1 * g7 * C[sg][1 % NO][r];
+        sp[o + 2 * sa.s_nd] = r1 * g7 * C[sg][2 % NO][r];
       } else {
 #pragma unroll
-        for (int q = 0; q < NQ; q++)
-          if (q < nc) sp[o + (long long)q * sa.s_nd] = g6 * C[sg][q][r];
+        for (int q = 0; q < NO; q++) sp[o + (long long)q * sa.s_nd] = g6 * C[sg][q][r];
       }
     }
   }
 }
 
 // ---------------------------------------------------------------------------
+// Host side.
 static int table_layouts(sht_plan *p, SplitState *st) {
   const int L = p->L, S = st->S;
   st->hb_off.assign(2 * (L + 1), 0);
@@ -685,6 +760,22 @@ static int table_layouts(sht_plan *p, SplitState *st) {
       fe += (long long)nchf * ip;
     }
   }
+  // data-operand slices: backward chunks per (slot, parity), forward
+  // (slot, chunk, sym/asym); sized for max_ncomp components
+  st->hcb.assign(2 * p->nslot, 0);
+  long long ch = 0;
+  for (int sl = 0; sl < p->nslot; sl++) {
+    const int M = sl >= L ? sl - L : L - sl, n = L - M + 1;
+    for (int par = 0; par < 2; par++) {
+      st->hcb[2 * sl + par] = ch;
+      ch += ((par ? n / 2 : (n + 1) / 2) + 63) / 64;
+    }
+  }
+  st->nchb = ch;
+  st->nchf = nchf;
+  const long long ntile = p->nr_p / 16;
+  const long long gb = ch * p->max_ncomp, gf = (long long)p->nslot * nchf * 2 * p->max_ncomp;
+  const long long groups = gb > gf ? gb : gf;
   auto alloc = [&](void **ptr, size_t bytes) -> int {
     if (hipMalloc(ptr, bytes ? bytes : 64) != hipSuccess) {
       set_error("split tables: allocation of %zu bytes failed", bytes);
@@ -701,13 +792,25 @@ static int table_layouts(sht_plan *p, SplitState *st) {
       (rc = alloc((void **)&st->beP, be * 4)) || (rc = alloc((void **)&st->bedP, be * 4)) ||
       (rc = alloc((void **)&st->fP, fo)) || (rc = alloc((void **)&st->fdP, fo)) ||
       (rc = alloc((void **)&st->fPa, fo)) || (rc = alloc((void **)&st->feP, fe * 4)) ||
-      (rc = alloc((void **)&st->fedP, fe * 4)) || (rc = alloc((void **)&st->fePa, fe * 4)))
+      (rc = alloc((void **)&st->fedP, fe * 4)) || (rc = alloc((void **)&st->fePa, fe * 4)) ||
+      (rc = alloc((void **)&st->cb, st->hcb.size() * sizeof(long long))) ||
+      (rc = alloc((void **)&st->dig, (size_t)(groups * S * ntile * 1024))) ||
+      (rc = alloc((void **)&st->dexp, (size_t)(groups * p->nr_p * 4))))
     return rc;
   KSHT_HIP(hipMemcpy(st->b_off, st->hb_off.data(), nb, hipMemcpyHostToDevice));
   KSHT_HIP(hipMemcpy(st->b_eoff, st->hb_eoff.data(), nb, hipMemcpyHostToDevice));
   KSHT_HIP(hipMemcpy(st->f_off, st->hf_off.data(), nb, hipMemcpyHostToDevice));
   KSHT_HIP(hipMemcpy(st->f_eoff, st->hf_eoff.data(), nb, hipMemcpyHostToDevice));
+  KSHT_HIP(hipMemcpy(st->cb, st->hcb.data(), st->hcb.size() * sizeof(long long),
+                     hipMemcpyHostToDevice));
   return SHT_OK;
+}
+
+template <int S>
+static constexpr int bwd_lds(int nb) { return 2 * 2 * nb * S * 1024; }
+template <int S>
+static constexpr int fwd_lds(int nb) {
+  return ((2 * 4 * nb * S <= 160) ? 2 : 1) * 4 * nb * S * 1024;
 }
 
 template <int S>
@@ -729,6 +832,18 @@ static int slice_tables(sht_plan *p, SplitState *st) {
   hipLaunchKernelGGL(k_slice_f<S>, gf, dim3(256), 0, p->stream, L, p->nth_h, p->nth_hp, p->t.offf,
                      p->t.ipad, p->t.Pf, p->t.asin_t, 1, st->f_off, st->f_eoff, st->fPa, st->fePa);
   KSHT_HIP(hipGetLastError());
+  // LDS limits of the GEMM kernels (dynamic shared memory above 64 KiB)
+#define KSHT_LDS(K, BYTES) \
+  KSHT_HIP(hipFuncSetAttribute((const void *)(K), hipFuncAttributeMaxDynamicSharedMemorySize, (BYTES)))
+  KSHT_LDS((k_leg_bwd_split<S, 3, true>), bwd_lds<S>(3));
+  KSHT_LDS((k_leg_bwd_split<S, 3, false>), bwd_lds<S>(3));
+  KSHT_LDS((k_leg_bwd_split<S, 2, false>), bwd_lds<S>(2));
+  KSHT_LDS((k_leg_bwd_split<S, 1, false>), bwd_lds<S>(1));
+  KSHT_LDS((k_leg_fwd_split<S, 3, true>), fwd_lds<S>(3));
+  KSHT_LDS((k_leg_fwd_split<S, 3, false>), fwd_lds<S>(3));
+  KSHT_LDS((k_leg_fwd_split<S, 2, false>), fwd_lds<S>(2));
+  KSHT_LDS((k_leg_fwd_split<S, 1, false>), fwd_lds<S>(1));
+#undef KSHT_LDS
   return SHT_OK;
 }
 
@@ -756,46 +871,74 @@ int split_init(sht_plan *p) {
 void split_free(sht_plan *p) {
   SplitState *st = (SplitState *)p->split;
   if (!st) return;
-  void *ptrs[] = {st->b_off, st->b_eoff, st->f_off, st->f_eoff, st->bP, st->bdP, st->beP, st->bedP,
-                  st->fP, st->fdP, st->fPa, st->feP, st->fedP, st->fePa};
+  void *ptrs[] = {st->b_off, st->b_eoff, st->f_off, st->f_eoff, st->bP,  st->bdP,
+                  st->beP,   st->bedP,   st->fP,    st->fdP,    st->fPa, st->feP,
+                  st->fedP,  st->fePa,   st->cb,    st->dig,    st->dexp};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   delete st;
   p->split = nullptr;
 }
 
-static SplitDims split_dims(const sht_plan *p, int nv, int ns, int nblk) {
+static SplitDims split_dims(const sht_plan *p, int ncomp, int nblk) {
   SplitDims d;
   d.L = p->L;
   d.nr = p->nr;
+  d.nr_p = p->nr_p;
   d.nth = p->nth;
   d.nth_h = p->nth_h;
   d.nth_hp = p->nth_hp;
-  d.nv = nv;
-  d.ns = ns;
+  d.ncomp = ncomp;
   d.n_rt = p->nr_p / 16;
   d.ncg = 0;
   d.nblk = nblk;
+  d.nchf = p->nth_hp / 64;
   return d;
 }
 
+// component groups of a call: vectors as (pol, dpol, tor) triples, scalars in
+// threes plus a remainder group of one or two
+struct GroupLaunch {
+  int nb, vec, ncg, nd_first;
+};
+static int component_groups(int nv, int ns, GroupLaunch g[3]) {
+  int n = 0;
+  if (nv) g[n++] = GroupLaunch{3, 1, nv, 0};
+  if (ns / 3) g[n++] = GroupLaunch{3, 0, ns / 3, 3 * nv};
+  if (ns % 3) g[n++] = GroupLaunch{ns % 3, 0, 1, 3 * nv + 3 * (ns / 3)};
+  return n;
+}
+
 template <int S>
-static int bwd_split_S(sht_plan *p, int nv, int ns, const double *sp, SpecAddr sa, double *out,
+static int bwd_split_S(sht_plan *p, int nv, int ns, const double *spec, SpecAddr sa, double *out,
                        GridAddr ga) {
   SplitState *st = (SplitState *)p->split;
+  const int ncomp = 3 * nv + ns;
+  hipLaunchKernelGGL(k_bslice<S>, dim3(p->nr_p, p->nslot), dim3(kSliceThreads), 0, p->stream,
+                     p->L, p->nr, p->nr_p, nv, ncomp, spec, sa, p->t.jst, st->cb, st->dig,
+                     st->dexp);
+  KSHT_HIP(hipGetLastError());
   SplitTabs tb{st->b_off, st->b_eoff, st->bP, st->bdP, nullptr, st->beP, st->bedP, nullptr};
-  const int nsg = (ns + kSplitScl - 1) / kSplitScl;
-  for (int vec = 1; vec >= 0; vec--) {
-    SplitDims d = split_dims(p, nv, ns, p->nth_hp / 32);
-    d.ncg = vec ? nv : nsg;
-    if (d.ncg == 0) continue;
-    const long long nb = (long long)(p->L + 1) * d.n_rt * d.ncg * d.nblk;
-    if (vec)
-      hipLaunchKernelGGL((k_leg_bwd_split<S, true>), dim3((unsigned)nb), dim3(kSplitThreads), 0, p->stream, d,
-                         0, sp, sa, out, ga, p->t.jst, tb, p->t.ipad, p->t.a_r, p->t.asin_t);
+  GroupLaunch g[3];
+  const int ng = component_groups(nv, ns, g);
+  for (int i = 0; i < ng; i++) {
+    SplitDims d = split_dims(p, ncomp, p->nth_hp / 64);
+    d.ncg = g[i].ncg;
+    const long long nblocks = (long long)(p->L + 1) * d.n_rt * d.ncg * d.nblk;
+    const dim3 grid((unsigned)nblocks), blk(kGemmThreads);
+#define KSHT_BWD(NB, VEC)                                                                      \
+  hipLaunchKernelGGL((k_leg_bwd_split<S, NB, VEC>), grid, blk, bwd_lds<S>(NB), p->stream, d, \
+                     g[i].nd_first, tb, st->cb, st->dig, st->dexp, out, ga, p->t.a_r,        \
+                     p->t.asin_t)
+    if (g[i].vec)
+      KSHT_BWD(3, true);
+    else if (g[i].nb == 3)
+      KSHT_BWD(3, false);
+    else if (g[i].nb == 2)
+      KSHT_BWD(2, false);
     else
-      hipLaunchKernelGGL((k_leg_bwd_split<S, false>), dim3((unsigned)nb), dim3(kSplitThreads), 0, p->stream,
-                         d, nv, sp, sa, out, ga, p->t.jst, tb, p->t.ipad, p->t.a_r, p->t.asin_t);
+      KSHT_BWD(1, false);
+#undef KSHT_BWD
     KSHT_HIP(hipGetLastError());
   }
   return SHT_OK;
@@ -805,20 +948,33 @@ template <int S>
 static int fwd_split_S(sht_plan *p, int nv, int ns, const double *in, GridAddr ga, double *sp,
                        SpecAddr sa) {
   SplitState *st = (SplitState *)p->split;
+  const int ncomp = 3 * nv + ns;
+  hipLaunchKernelGGL(k_fslice<S>, dim3(p->nr_p / 16, p->nslot), dim3(kSliceThreads), 0,
+                     p->stream, p->nr, p->nr_p, p->nth, p->nth_h, st->nchf, ncomp, in, ga,
+                     p->t.weight, st->dig, st->dexp);
+  KSHT_HIP(hipGetLastError());
   SplitTabs tb{st->f_off, st->f_eoff, st->fP, st->fdP, st->fPa, st->feP, st->fedP, st->fePa};
-  const int nsg = (ns + kSplitScl - 1) / kSplitScl;
   const int ne_max = (p->L + 2) / 2;
-  for (int vec = 1; vec >= 0; vec--) {
-    SplitDims d = split_dims(p, nv, ns, (ne_max + 31) / 32);
-    d.ncg = vec ? nv : nsg;
-    if (d.ncg == 0) continue;
-    const long long nb = (long long)(p->L + 1) * d.n_rt * d.ncg * d.nblk;
-    if (vec)
-      hipLaunchKernelGGL((k_leg_fwd_split<S, true>), dim3((unsigned)nb), dim3(kSplitThreads), 0, p->stream, d,
-                         0, in, ga, sp, sa, p->t.jst, tb, p->t.ipad, p->t.radius, p->t.weight);
+  GroupLaunch g[3];
+  const int ng = component_groups(nv, ns, g);
+  for (int i = 0; i < ng; i++) {
+    SplitDims d = split_dims(p, ncomp, (ne_max + 63) / 64);
+    d.ncg = g[i].ncg;
+    const long long nblocks = (long long)(p->L + 1) * d.n_rt * d.ncg * d.nblk;
+    const dim3 grid((unsigned)nblocks), blk(kGemmThreads);
+#define KSHT_FWD(NB, VEC)                                                                      \
+  hipLaunchKernelGGL((k_leg_fwd_split<S, NB, VEC>), grid, blk, fwd_lds<S>(NB), p->stream, d, \
+                     g[i].nd_first, tb, st->dig, st->dexp, sp, sa, p->t.jst, p->t.ipad,      \
+                     p->t.radius)
+    if (g[i].vec)
+      KSHT_FWD(3, true);
+    else if (g[i].nb == 3)
+      KSHT_FWD(3, false);
+    else if (g[i].nb == 2)
+      KSHT_FWD(2, false);
     else
-      hipLaunchKernelGGL((k_leg_fwd_split<S, false>), dim3((unsigned)nb), dim3(kSplitThreads), 0, p->stream,
-                         d, nv, in, ga, sp, sa, p->t.jst, tb, p->t.ipad, p->t.radius, p->t.weight);
+      KSHT_FWD(1, false);
+#undef KSHT_FWD
     KSHT_HIP(hipGetLastError());
   }
   return SHT_OK;
@@ -829,6 +985,11 @@ int launch_leg_bwd_split(sht_plan *p, int nv, int ns, const double *sp, SpecAddr
   if (!p->split) {
     set_error("split precision: plan was not created in split mode");
     return SHT_ERR_STATE;
+  }
+  if (3 * nv + ns > p->max_ncomp) {
+    set_error("split precision: %d components exceed the plan's max_ncomp %d", 3 * nv + ns,
+              p->max_ncomp);
+    return SHT_ERR_ARG;
   }
   switch (((SplitState *)p->split)->S) {
     case 4: return bwd_split_S<4>(p, nv, ns, sp, sa, out, ga);
@@ -843,6 +1004,11 @@ int launch_leg_fwd_split(sht_plan *p, int nv, int ns, const double *in, GridAddr
   if (!p->split) {
     set_error("split precision: plan was not created in split mode");
     return SHT_ERR_STATE;
+  }
+  if (3 * nv + ns > p->max_ncomp) {
+    set_error("split precision: %d components exceed the plan's max_ncomp %d", 3 * nv + ns,
+              p->max_ncomp);
+    return SHT_ERR_ARG;
   }
   switch (((SplitState *)p->split)->S) {
     case 4: return fwd_split_S<4>(p, nv, ns, in, ga, sp, sa);

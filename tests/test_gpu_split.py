@@ -16,7 +16,7 @@ CASES = {
     "L15_nr5_nth25_odd": ShellConfig(15, 5, 25, 40),
     "L127_nr9_nth192": ShellConfig(127, 9, 192, 384),
 }
-TOL = {4: 1e-7, 5: 1e-9, 6: 1e-11}
+TOL = {4: 1e-6, 5: 1e-8, 6: 1e-11, 7: 1e-13}
 
 
 def rel_comp(a, b, ncomp, comp_fastest=True):
@@ -32,7 +32,7 @@ def case(request, torch_cuda):
     return request.param, cfg, orc
 
 
-@pytest.mark.parametrize("S", [4, 5, 6])
+@pytest.mark.parametrize("S", [4, 5, 6, 7])
 @pytest.mark.parametrize("slope", [1.5, 3.0])
 def test_split_roundtrip_vs_oracle(case, torch_cuda, S, slope):
     name, cfg, orc = case
@@ -74,5 +74,31 @@ def test_split_stage_scalars(case, torch_cuda):
         t.cuda.synchronize()
         assert rel_comp(out.cpu().numpy(), ref, 5) <= TOL[6]
         assert rel_comp(outf.cpu().numpy(), ref_f, 5) <= TOL[6]
+    finally:
+        plan.close()
+
+
+@pytest.mark.parametrize("nv,ns", [(1, 1), (3, 4), (0, 3)])
+def test_split_component_groups(case, torch_cuda, nv, ns):
+    """Vector groups and scalar groups of three, two and one component."""
+    name, cfg, orc = case
+    t = torch_cuda
+    nc = 3 * nv + ns
+    print(f"[gpu] split groups {name} nv={nv} ns={ns}", flush=True)
+    plan = SphTransPlan(cfg, max_ncomp=nc, precision=SHT_PREC_SPLIT_INT8, n_slices=6)
+    try:
+        sp = synthetic_spectra(cfg, nv, ns, seed=37)
+        ref = orc.leg_bwd(nv, ns, sp)
+        out = t.zeros(cfg.n_rtm(nc), dtype=t.float64, device="cuda")
+        plan.legendre_backward(nv, ns, t.from_numpy(sp).cuda(), out)
+        vr = np.random.default_rng(41).standard_normal(cfg.n_rtm(nc))
+        ref_f = orc.leg_fwd(nv, ns, vr)
+        outf = t.zeros(cfg.n_rj(nc), dtype=t.float64, device="cuda")
+        plan.legendre_forward(nv, ns, t.from_numpy(vr).cuda(), outf)
+        t.cuda.synchronize()
+        eb = rel_comp(out.cpu().numpy(), ref, nc)
+        ef = rel_comp(outf.cpu().numpy(), ref_f, nc)
+        print(f"    backward max rel L2 {eb:.3e}  forward max rel L2 {ef:.3e}", flush=True)
+        assert eb <= TOL[6] and ef <= TOL[6]
     finally:
         plan.close()

