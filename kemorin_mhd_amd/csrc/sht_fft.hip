@@ -9,6 +9,7 @@
 // 5 and a direct DFT for other primes up to 13) in LDS.  The Fourier side
 // holds orders m <= L in the compact order q = 0 (a0), 2m-1 (a_m), 2m (b_m);
 // the grid side is the reference's field-major v_rtp with phi slowest.
+#include <algorithm>
 #include <cmath>
 
 #include "sht_internal.h"
@@ -278,6 +279,261 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_fwd(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Split form for Nphi = R0 * NP with NP a power of two, L < NP (the usual
+// Nphi = 3 * 2^k grids).  A work-group owns 16 consecutive lines (8 complex
+// lines of two packed real lines) so every global access moves whole
+// 128-byte rows of the line-contiguous layouts.  With j = R0 j2 + j1:
+//   backward  x_j = Re/Im of sum_m Z_m e^{+2 pi i m j/N}
+//                 = sum_{m'<NP} D^{(j1)}_m' e^{+2 pi i m' j2/NP},
+//             D^{(j1)}_m' = e^{2 pi i m' j1/N} Z_m' + e^{2 pi i (m'+N-NP) j1/N} Z_{m'+N-NP}
+//   (Z_m is non-zero only for m <= L and m >= N-L, so only these two of the
+//   R0 aliases of m' survive), i.e. R0 NP-point FFTs per line; the forward
+//   runs the same steps in reverse and accumulates Z_m (m <= L) and Z_{N-m}.
+// The NP-point FFTs are Stockham radix-8 passes (radix 4 / 2 to finish) with
+// eight points per thread in registers and padded LDS between passes; input
+// and output rows are staged through LDS (row stride 17 doubles).
+constexpr int kStageStride = 17;  // doubles per staged row of 16 lines
+
+__device__ __forceinline__ int fpad(int i) { return i + (i >> 3); }
+
+template <int NP, int Ns>
+__device__ __forceinline__ void f3_passes(double2 (&v)[8], double2 *ln, int j,
+                                          const double2 *__restrict__ twn, int N, int R0,
+                                          double s) {
+  constexpr int T = NP / 8;
+  constexpr int R = (NP / Ns >= 8) ? 8 : NP / Ns;
+#pragma unroll
+  for (int i = 0; i < 8 / R; i++) {
+    const int b = j + T * i;
+    if constexpr (Ns > 1) {
+      const int k = b % Ns;
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        // exp(-2 pi i k r / (Ns R)) = twn[k r (NP / (Ns R)) R0]
+        double2 w = twn[(k * r * (NP / (Ns * R)) * R0) % N];
+        w.y *= -s;
+        v[i * R + r] = cmul(v[i * R + r], w);
+      }
+    }
+    dft_small<R>(v + i * R, s);
+  }
+  if constexpr (Ns * R < NP) {
+#pragma unroll
+    for (int i = 0; i < 8 / R; i++) {
+      const int b = j + T * i;
+#pragma unroll
+      for (int r = 0; r < R; r++) ln[fpad((b / Ns) * Ns * R + b % Ns + Ns * r)] = v[i * R + r];
+    }
+    __syncthreads();
+    constexpr int R2 = (NP / (Ns * R) >= 8) ? 8 : NP / (Ns * R);
+#pragma unroll
+    for (int i = 0; i < 8 / R2; i++)
+#pragma unroll
+      for (int r = 0; r < R2; r++) v[i * R2 + r] = ln[fpad(j + T * i + (NP / R2) * r)];
+    __syncthreads();
+    f3_passes<NP, Ns * R>(v, ln, j, twn, N, R0, s);
+  }
+}
+
+// radix of the last pass, and the register holding output slot q (index j + T q)
+template <int NP, int Ns = 1>
+constexpr int f3_last_radix() {
+  constexpr int R = (NP / Ns >= 8) ? 8 : NP / Ns;
+  if constexpr (Ns * R < NP)
+    return f3_last_radix<NP, Ns * R>();
+  else
+    return R;
+}
+template <int NP>
+__device__ __forceinline__ constexpr int f3_pos(int q) {
+  constexpr int R = f3_last_radix<NP>();
+  return (q % (8 / R)) * R + q / (8 / R);
+}
+
+__device__ __forceinline__ double2 cfma(double2 a, double2 w, double2 acc) {
+  return make_double2(acc.x + a.x * w.x - a.y * w.y, acc.y + a.x * w.y + a.y * w.x);
+}
+
+template <int NP>
+__global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long nline,
+                                                 const double *__restrict__ X, FourierAddr xa,
+                                                 double *__restrict__ v,
+                                                 const double2 *__restrict__ twn) {
+  extern __shared__ double2 lds[];
+  constexpr int T = NP / 8;
+  const int tid = threadIdx.x, cl = tid / T, j = tid % T;
+  const int nd = blockIdx.y;
+  const long long line0 = (long long)xcd_block(blockIdx.x, gridDim.x) * 16;
+  double *stage = reinterpret_cast<double *>(lds);
+  double2 zlo[8], zhi[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) zlo[q] = zhi[q] = make_double2(0.0, 0.0);
+  // coefficients in chunks of 256 orders (rows 2 f0 - 1 .. 2 f0 + 510)
+  const long long xbase = (long long)nd * xa.s_nd;
+  for (int f0 = 0; f0 <= L; f0 += 256) {
+    const int f1 = min(f0 + 256, L + 1);
+    const int q0 = f0 == 0 ? 0 : 2 * f0 - 1, nrow = 2 * (f1 - 1) - q0 + 1;
+    for (int idx = tid; idx < nrow * 16; idx += NP) {
+      const int row = idx >> 4, l16 = idx & 15;
+      const long long line = line0 + l16;
+      stage[row * kStageStride + l16] =
+          line < nline ? X[xa.q_off[q0 + row] * xa.q_mul + xbase + line * xa.s_line] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int f = j + T * q;  // order of Z_f; Z_{f+N-NP} comes from order NP - f
+      if (f >= f0 && f < f1) {
+        const double *p = stage + ((f == 0 ? 0 : 2 * f - 1) - q0) * kStageStride + 2 * cl;
+        if (f == 0) {
+          zlo[q] = make_double2(p[0], p[1]);
+        } else {
+          const double aa = p[0], ab = p[1], ba = p[kStageStride], bb = p[kStageStride + 1];
+          zlo[q] = make_double2(0.5 * (aa + bb), 0.5 * (ab - ba));
+        }
+      }
+      const int g = NP - f;
+      if (g >= f0 && g < f1 && g >= 1) {
+        const double *p = stage + (2 * g - 1 - q0) * kStageStride + 2 * cl;
+        const double aa = p[0], ab = p[1], ba = p[kStageStride], bb = p[kStageStride + 1];
+        zhi[q] = make_double2(0.5 * (aa - bb), 0.5 * (ab + ba));
+      }
+    }
+    __syncthreads();
+  }
+  double2 *ln = lds + cl * (NP + NP / 8);
+  const long long nnod = nline * N;
+  const bool vec2 = (nline & 1) == 0;
+  for (int j1 = 0; j1 < R0; j1++) {
+    double2 w[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int m = j + T * q;
+      double2 t1 = twn[(int)(((long long)m * j1) % N)];
+      double2 t2 = twn[(int)(((long long)(m + N - NP) * j1) % N)];
+      t1.y = -t1.y;  // e^{+...}
+      t2.y = -t2.y;
+      w[q] = cfma(zhi[q], t2, cmul(zlo[q], t1));
+    }
+    f3_passes<NP, 1>(w, ln, j, twn, N, R0, 1.0);
+    // rows phi = R0 (j + T q) + j1, staged as [j2][8 complex lines] (stride 9)
+#pragma unroll
+    for (int q = 0; q < 8; q++) lds[(j + T * q) * 9 + cl] = w[f3_pos<NP>(q)];
+    __syncthreads();
+    for (int idx = tid; idx < NP * 8; idx += NP) {
+      const int j2 = idx >> 3, c = idx & 7;
+      const long long la = line0 + 2 * c;
+      const double2 z = lds[j2 * 9 + c];
+      double *o = v + (long long)nd * nnod + (long long)(R0 * j2 + j1) * nline + la;
+      if (la + 1 < nline) {
+        if (vec2)
+          *reinterpret_cast<double2 *>(o) = z;
+        else {
+          o[0] = z.x;
+          o[1] = z.y;
+        }
+      } else if (la < nline) {
+        o[0] = z.x;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int NP>
+__global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long nline,
+                                                 const double *__restrict__ v,
+                                                 double *__restrict__ X, FourierAddr xa,
+                                                 const double2 *__restrict__ twn) {
+  extern __shared__ double2 lds[];
+  constexpr int T = NP / 8;
+  const int tid = threadIdx.x, cl = tid / T, j = tid % T;
+  const int nd = blockIdx.y;
+  const long long line0 = (long long)xcd_block(blockIdx.x, gridDim.x) * 16;
+  const long long nnod = nline * N;
+  const bool vec2 = (nline & 1) == 0;
+  double2 zlo[8], zhi[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) zlo[q] = zhi[q] = make_double2(0.0, 0.0);
+  double2 *ln = lds + cl * (NP + NP / 8);
+  for (int j1 = 0; j1 < R0; j1++) {
+    for (int idx = tid; idx < NP * 8; idx += NP) {
+      const int j2 = idx >> 3, c = idx & 7;
+      const long long la = line0 + 2 * c;
+      const double *src = v + (long long)nd * nnod + (long long)(R0 * j2 + j1) * nline + la;
+      double2 z = make_double2(0.0, 0.0);
+      if (la + 1 < nline) {
+        z = vec2 ? *reinterpret_cast<const double2 *>(src) : make_double2(src[0], src[1]);
+      } else if (la < nline) {
+        z.x = src[0];
+      }
+      lds[j2 * 9 + c] = z;
+    }
+    __syncthreads();
+    double2 w[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) w[q] = lds[(j + T * q) * 9 + cl];
+    __syncthreads();
+    f3_passes<NP, 1>(w, ln, j, twn, N, R0, -1.0);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int k = j + T * q;
+      const double2 y = w[f3_pos<NP>(q)];
+      if (k <= L) zlo[q] = cfma(y, twn[(int)(((long long)k * j1) % N)], zlo[q]);
+      if (k >= NP - L)
+        zhi[q] = cfma(y, twn[(int)(((long long)(k + N - NP) * j1) % N)], zhi[q]);
+    }
+  }
+  // Z_{N-m} for m = j + T q sits at index NP - m of the lane (T - j) % T of
+  // this complex line, in register 7 - q (or (8 - q) % 8 when j = 0)
+  const int lane = threadIdx.x & 63;
+  const int partner = lane - j + (T - j) % T;
+  double2 zn[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const double2 send = (j == 0) ? zhi[(8 - q) % 8] : zhi[7 - q];
+    zn[q].x = __shfl(send.x, partner, 64);
+    zn[q].y = __shfl(send.y, partner, 64);
+  }
+  const double sn = 1.0 / (double)N, tsn = 2.0 / (double)N;
+  double *stage = reinterpret_cast<double *>(lds);
+  const long long xbase = (long long)nd * xa.s_nd;
+  for (int f0 = 0; f0 <= L; f0 += 256) {
+    const int f1 = min(f0 + 256, L + 1);
+    const int q0 = f0 == 0 ? 0 : 2 * f0 - 1, nrow = 2 * (f1 - 1) - q0 + 1;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int m = j + T * q;
+      if (m >= f0 && m < f1) {
+        const double2 zm = zlo[q], zc = (m == 0) ? zlo[q] : zn[q];
+        // X_a = (Z_m + conj Z_{N-m})/2, X_b = -i (Z_m - conj Z_{N-m})/2
+        const double xar = 0.5 * (zm.x + zc.x), xai = 0.5 * (zm.y - zc.y);
+        const double xbr = 0.5 * (zm.y + zc.y), xbi = -0.5 * (zm.x - zc.x);
+        if (m == 0) {
+          double *p = stage + 2 * cl;
+          p[0] = sn * xar;
+          p[1] = sn * xbr;
+        } else {
+          double *p = stage + (2 * m - 1 - q0) * kStageStride + 2 * cl;
+          p[0] = tsn * xar;
+          p[1] = tsn * xbr;
+          p[kStageStride] = -tsn * xai;
+          p[kStageStride + 1] = -tsn * xbi;
+        }
+      }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < nrow * 16; idx += NP) {
+      const int row = idx >> 4, l16 = idx & 15;
+      const long long line = line0 + l16;
+      if (line < nline)
+        X[xa.q_off[q0 + row] * xa.q_mul + xbase + line * xa.s_line] = stage[row * kStageStride + l16];
+    }
+  }
+}
+
 int fft_plan_init(sht_plan *p) {
   FftPlan &fp = p->fft;
   const int n = p->nphi;
@@ -336,8 +592,56 @@ int fft_plan_init(sht_plan *p) {
   KSHT_HIP(hipMalloc(&p->t.tw, sizeof(double2) * tw.size()));
   KSHT_HIP(hipMemcpy(p->t.tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice));
   p->device_bytes += (long long)(sizeof(double2) * tw.size());
+  // split form n = r0 * np: np the smallest power of two above L, 32..512
+  fp.fast = 0;
+  int np = 32;
+  while (np < p->L + 1) np *= 2;
+  if (np <= 512 && n % np == 0) {
+    fp.fast = 1;
+    fp.np = np;
+    fp.r0 = n / np;
+    const int rows = 2 * std::min(256, p->L + 1);
+    fp.lds3 = (int)std::max({sizeof(double) * rows * kStageStride,
+                             sizeof(double2) * 8 * (np + np / 8), sizeof(double2) * np * 9});
+    std::vector<double2> twn(n);
+    for (int k = 0; k < n; k++) {
+      const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+      twn[k] = make_double2((double)cosl(a), (double)sinl(a));
+    }
+    KSHT_HIP(hipMalloc(&p->t.twn, sizeof(double2) * n));
+    KSHT_HIP(hipMemcpy(p->t.twn, twn.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+    p->device_bytes += (long long)(sizeof(double2) * n);
+#define KSHT_FFT3_ATTR(NP)                                                                   \
+  case NP:                                                                                   \
+    KSHT_HIP(hipFuncSetAttribute((const void *)k_fft3_bwd<NP>,                               \
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, fp.lds3));      \
+    KSHT_HIP(hipFuncSetAttribute((const void *)k_fft3_fwd<NP>,                               \
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, fp.lds3));      \
+    break;
+    switch (np) {
+      KSHT_FFT3_ATTR(32)
+      KSHT_FFT3_ATTR(64)
+      KSHT_FFT3_ATTR(128)
+      KSHT_FFT3_ATTR(256)
+      KSHT_FFT3_ATTR(512)
+      default: break;
+    }
+#undef KSHT_FFT3_ATTR
+  }
   return SHT_OK;
 }
+
+#define KSHT_FFT3_LAUNCH(K, ...)                                                             \
+  do {                                                                                     \
+    const dim3 g3((unsigned)((nline + 15) / 16), (unsigned)ncomp);                         \
+    switch (p->fft.np) {                                                                   \
+      case 32: hipLaunchKernelGGL((K<32>), g3, dim3(32), p->fft.lds3, p->stream, __VA_ARGS__); break; \
+      case 64: hipLaunchKernelGGL((K<64>), g3, dim3(64), p->fft.lds3, p->stream, __VA_ARGS__); break; \
+      case 128: hipLaunchKernelGGL((K<128>), g3, dim3(128), p->fft.lds3, p->stream, __VA_ARGS__); break; \
+      case 256: hipLaunchKernelGGL((K<256>), g3, dim3(256), p->fft.lds3, p->stream, __VA_ARGS__); break; \
+      default: hipLaunchKernelGGL((K<512>), g3, dim3(512), p->fft.lds3, p->stream, __VA_ARGS__); break; \
+    }                                                                                      \
+  } while (0)
 
 int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, double *v) {
   const long long nline = (long long)p->nr * p->nth;
@@ -358,7 +662,10 @@ int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, doub
         hipLaunchKernelGGL((K<kFftElemsLarge, false>), grid, dim3(kFftThreads), lds, p->stream, __VA_ARGS__); \
     }                                                                                      \
   } while (0)
-  KSHT_FFT_LAUNCH(k_fft_bwd, p->fft, p->L, nline, X, xa, v, p->t.tw);
+  if (p->fft.fast)
+    KSHT_FFT3_LAUNCH(k_fft3_bwd, p->nphi, p->fft.r0, p->L, nline, X, xa, v, p->t.twn);
+  else
+    KSHT_FFT_LAUNCH(k_fft_bwd, p->fft, p->L, nline, X, xa, v, p->t.tw);
   KSHT_HIP(hipGetLastError());
   return SHT_OK;
 }
@@ -368,7 +675,10 @@ int launch_fft_fwd(sht_plan *p, int ncomp, const double *v, double *X, FourierAd
   const int B = p->fft.lines_per_wg;
   const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);
   const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.n;
-  KSHT_FFT_LAUNCH(k_fft_fwd, p->fft, p->L, nline, v, X, xa, p->t.tw);
+  if (p->fft.fast)
+    KSHT_FFT3_LAUNCH(k_fft3_fwd, p->nphi, p->fft.r0, p->L, nline, v, X, xa, p->t.twn);
+  else
+    KSHT_FFT_LAUNCH(k_fft_fwd, p->fft, p->L, nline, v, X, xa, p->t.tw);
   KSHT_HIP(hipGetLastError());
   return SHT_OK;
 }

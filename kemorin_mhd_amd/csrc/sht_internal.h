@@ -40,6 +40,8 @@ struct FftPass {
 struct FftPlan {
   int n, npass, lines_per_wg, pad;  // pad: LDS elements per work-group
   FftPass pass[kMaxPass];
+  // split form n = r0 * np (np a power of two above L): 16 lines per work-group
+  int fast, np, r0, lds3;
 };
 
 // Addressing of an (order m, colatitude t, radius k, component nd) element
@@ -74,6 +76,7 @@ struct Tables {
   double *weight = nullptr, *asin_t = nullptr;  // device, Ntheta
   double *cos_t = nullptr, *sin_t = nullptr;    // device, Ntheta
   double2 *tw = nullptr;                        // FFT twiddles
+  double2 *twn = nullptr;                       // exp(-2 pi i k / Nphi), k < Nphi
   long long *slot_off_X = nullptr, *slot_off_rtm = nullptr;  // 2L+1 each
   long long *q_off_X = nullptr, *q_off_rtp = nullptr;        // 2L+1 each
 };

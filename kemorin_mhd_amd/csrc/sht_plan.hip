@@ -225,7 +225,7 @@ static void free_plan(sht_plan *p) {
   if (!p) return;
   void *ptrs[] = {p->t.offb, p->t.offf, p->t.ipad, p->t.jst, p->t.Pb, p->t.dPb,
                   p->t.Pf, p->t.dPf, p->t.radius, p->t.a_r, p->t.weight, p->t.asin_t,
-                  p->t.cos_t, p->t.sin_t, p->t.tw, p->t.slot_off_X, p->t.slot_off_rtm,
+                  p->t.cos_t, p->t.sin_t, p->t.tw, p->t.twn, p->t.slot_off_X, p->t.slot_off_rtm,
                   p->t.q_off_X, p->t.q_off_rtp, p->X};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);

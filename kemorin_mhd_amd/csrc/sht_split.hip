@@ -34,6 +34,26 @@ namespace ksht {
 
 constexpr int kEmpty = -4000;  // exponent of an all-zero row / column
 
+// Backward reduction chunks over the pair index i (l = M + 2i + parity): 64
+// wide, except that orders M < kNarrowM start with [0, 16) and [16, 64)
+// (zero-padded to 64 K lanes).  At low orders the spectra fall and P/sin, dP
+// grow across l = M .. M+127 by orders of magnitude; per-chunk exponents over
+// narrower degree ranges keep the sliced products accurate there.
+constexpr int kNarrowM = 64;
+__host__ __device__ __forceinline__ int bchunk_count(int M, int np) {
+  if (np <= 0) return 0;
+  if (M >= kNarrowM) return (np + 63) / 64;
+  return np <= 16 ? 1 : np <= 64 ? 2 : 1 + (np + 63) / 64;
+}
+__host__ __device__ __forceinline__ int bchunk_start(int M, int c) {
+  if (M >= kNarrowM) return 64 * c;
+  return c == 0 ? 0 : c == 1 ? 16 : 64 * (c - 1);
+}
+__host__ __device__ __forceinline__ int bchunk_end(int M, int c) {
+  if (M >= kNarrowM) return 64 * c + 64;
+  return c == 0 ? 16 : c == 1 ? 64 : 64 * c;
+}
+
 struct SplitState {
   int S;
   // backward tables: rows theta (nth_hp), K = degree pair index i, chunk 64
@@ -114,18 +134,19 @@ __global__ void __launch_bounds__(256) k_slice_b(int L, int nth_hp, const long l
                                                  signed char *out, int *eout) {
   const int M = blockIdx.z >> 1, par = blockIdx.z & 1;
   const int n = L - M + 1, np = par ? n / 2 : (n + 1) / 2;
-  const int nch = (np + 63) / 64;
+  const int nch = bchunk_count(M, np);
   const int c = blockIdx.y;
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (c >= nch || t >= nth_hp) return;
   const int ip = ipad[2 * M + par];
+  const int i0 = bchunk_start(M, c), i1 = min(bchunk_end(M, c), ip);
   const double *src = P + offb[2 * M + par] + t;
   double v[64];
   int e = kEmpty;
 #pragma unroll
   for (int q = 0; q < 64; q++) {
-    const int i = c * 64 + q;
-    v[q] = i < ip ? src[(long long)i * nth_hp] : 0.0;
+    const int i = i0 + q;
+    v[q] = i < i1 ? src[(long long)i * nth_hp] : 0.0;
     e = max(e, exp_of(v[q]));
   }
   eout[eoff[2 * M + par] + (long long)c * nth_hp + t] = e;
@@ -222,57 +243,70 @@ __device__ __forceinline__ void put_digits(signed char *blk, long long sstride, 
 constexpr int kSliceThreads = 256;
 
 // ---------------------------------------------------------------------------
-// Backward data operand.  Work-group = (radius k < nr_p, slot m + L); the
-// slot's spectra at radius k are one contiguous run of n * ncomp doubles.
-// Wave task = (chunk of 64 pair indices over both parities, component):
-// lane = pair index i of l = M + 2i + parity.  The wave takes the chunk's
-// exponent, writes S digit bytes per lane in fragment order and the exponent.
-// The first component of each vector (pol) is scaled by g3 = l(l+1) (1/2 at
+// Backward data operand.  Work-group = (radius k < nr_p, slot m + L); per
+// chunk of pair indices [i0, i1) the slot's spectra at radius k for degrees
+// l - M in [2 i0, 2 i1) (one contiguous run of at most 128 ncomp doubles)
+// are staged in LDS with coalesced loads.  Wave task = (parity, component): lane
+// = pair index i of l = M + 2i + parity; the wave takes the chunk exponent
+// and writes S digit bytes per lane in fragment order plus the exponent.  The
+// first component of each vector (pol) is scaled by g3 = l(l+1) (1/2 at
 // l = 0) as in the FP64 kernels; radii nr <= k < nr_p get zero digits.
 template <int S>
 __global__ void __launch_bounds__(kSliceThreads) k_bslice(
     int L, int nr, int nr_p, int nv, int ncomp, const double *__restrict__ sp, SpecAddr sa,
     const int *__restrict__ jst, const long long *__restrict__ cb, signed char *__restrict__ dig,
     int *__restrict__ dexp) {
+  extern __shared__ double run[];  // [128 degrees][ncomp]
   const int k = blockIdx.x, sl = blockIdx.y;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int M = sl >= L ? sl - L : L - sl;
   const int n = L - M + 1, np0 = (n + 1) / 2, np1 = n / 2;
-  const int nch0 = (np0 + 63) / 64, nch1 = (np1 + 63) / 64;
+  const int nch0 = bchunk_count(M, np0), nch1 = bchunk_count(M, np1);
   const int ntile = nr_p >> 4;
   const int off = frag_off(k & 15, lane);
   const long long sstride = (long long)ntile * 1024;
   const double *src = sp + (long long)jst[sl] * sa.s_j + (long long)k * sa.s_k;
-  for (int task = wave; task < (nch0 + nch1) * ncomp; task += kSliceThreads / 64) {
-    const int cq = task % ncomp, cc = task / ncomp;
-    const int par = cc >= nch0 ? 1 : 0, c = par ? cc - nch0 : cc;
-    const int i = c * 64 + lane;
-    double x = 0.0;
-    if (k < nr && i < (par ? np1 : np0)) {
-      x = src[(long long)(par + 2 * i) * sa.s_j + (long long)cq * sa.s_nd];
-      if (cq < 3 * nv && cq % 3 == 0) x *= g3s_of(M + 2 * i + par);
+  for (int c = 0; c < nch0; c++) {
+    const int i0 = bchunk_start(M, c), ni = bchunk_end(M, c) - i0;
+    const int d0 = 2 * i0, nd_run = min(2 * ni, n - d0) * ncomp;
+    __syncthreads();
+    for (int idx = tid; idx < nd_run; idx += kSliceThreads) {
+      const int dd = idx / ncomp, cq = idx - dd * ncomp;
+      run[idx] = k < nr ? src[(long long)(d0 + dd) * sa.s_j + (long long)cq * sa.s_nd] : 0.0;
     }
-    const int e = wave_max(exp_of(x));
-    const long long gb = (cb[2 * sl + par] + c) * ncomp + cq;
-    put_digits<S>(dig + (gb * S * ntile + (k >> 4)) * 1024, sstride, off, fix_of<S>(x, e));
-    if (lane == 0) dexp[gb * nr_p + k] = e;
+    __syncthreads();
+    const int ntask = (c < nch1 ? 2 : 1) * ncomp;
+    for (int task = wave; task < ntask; task += kSliceThreads / 64) {
+      const int par = task >= ncomp ? 1 : 0, cq = task - par * ncomp;
+      const int i = i0 + lane;
+      double x = 0.0;
+      if (lane < ni && i < (par ? np1 : np0)) {
+        x = run[(2 * lane + par) * ncomp + cq];
+        if (cq < 3 * nv && cq % 3 == 0) x *= g3s_of(M + 2 * i + par);
+      }
+      const int e = wave_max(exp_of(x));
+      const long long gb = (cb[2 * sl + par] + c) * ncomp + cq;
+      put_digits<S>(dig + (gb * S * ntile + (k >> 4)) * 1024, sstride, off, fix_of<S>(x, e));
+      if (lane == 0) dexp[gb * nr_p + k] = e;
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
 // Forward data operand.  Work-group = (radius tile of 16, slot m + L).  Per
-// (chunk of 64 hemisphere rows, component) the tile's rows are staged in LDS
-// (loads ordered along the smaller of the radius and colatitude strides) as
-// the weighted symmetric and antisymmetric combinations
-// (v(t) +- v(nth-1-t)) w(t), as in the FP64 kernel (an odd grid's equator row
-// is its own mirror: (v w, 0)); wave w then slices radii 4w .. 4w+3 with
-// lane = row.
+// chunk of 64 hemisphere rows and group of up to kFsGroup components the
+// tile's north and mirrored south rows are staged in LDS (loads ordered along
+// the smaller of the radius and colatitude strides) as the weighted symmetric
+// and antisymmetric combinations (v(t) +- v(nth-1-t)) w(t), as in the FP64
+// kernel (an odd grid's equator row is its own mirror: (v w, 0)); each wave
+// then slices (component, radius, sym/asym) columns with lane = row.
+constexpr int kFsGroup = 3;
 template <int S>
 __global__ void __launch_bounds__(kSliceThreads) k_fslice(
     int nr, int nr_p, int nth, int nth_h, int nchf, int ncomp, const double *__restrict__ in,
     GridAddr ga, const double *__restrict__ weight, signed char *__restrict__ dig,
     int *__restrict__ dexp) {
-  __shared__ double sSym[64][17], sAsym[64][17];
+  __shared__ double sC[kFsGroup][2][64][17];  // [component][sym/asym][row][radius]
   const int kt = blockIdx.x, sl = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ntile = nr_p >> 4;
@@ -281,58 +315,64 @@ __global__ void __launch_bounds__(kSliceThreads) k_fslice(
   const long long base = ga.slot_off[sl] * ga.slot_mul;
   const long long sstride = (long long)ntile * 1024;
   for (int c = 0; c < nchf; c++) {
-    for (int cq = 0; cq < ncomp; cq++) {
+    for (int cg = 0; cg < ncomp; cg += kFsGroup) {
+      const int ng = min(kFsGroup, ncomp - cg);
+      __syncthreads();
+      double vn[kFsGroup * 4], vs[kFsGroup * 4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int idx = tid + kSliceThreads * u;
+      for (int u = 0; u < kFsGroup * 4; u++) {
+        const int f = u / 4, idx = tid + kSliceThreads * (u % 4);
         const int kk = k_fast ? (idx & 15) : (idx >> 6);
         const int tt = k_fast ? (idx >> 4) : (idx & 63);
         const int t = c * 64 + tt, k = kt * 16 + kk;
-        double xs = 0.0, xa = 0.0;
-        if (k < nr && t < nth_h) {
-          const long long b = base + (long long)cq * ga.s_nd + (long long)k * ga.s_k;
-          const double vn = in[b + (long long)t * ga.s_t];
-          const double vs = t != t_eq ? in[b + (long long)(nth - 1 - t) * ga.s_t] : 0.0;
-          const double w = weight[t];
-          xs = (vn + vs) * w;
-          xa = t != t_eq ? (vn - vs) * w : 0.0;
+        vn[u] = vs[u] = 0.0;
+        if (f < ng && k < nr && t < nth_h) {
+          const long long b = base + (long long)(cg + f) * ga.s_nd + (long long)k * ga.s_k;
+          vn[u] = in[b + (long long)t * ga.s_t];
+          if (t != t_eq) vs[u] = in[b + (long long)(nth - 1 - t) * ga.s_t];
         }
-        sSym[tt][kk] = xs;
-        sAsym[tt][kk] = xa;
       }
-      __syncthreads();
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int kk = wave * 4 + j;
-        const double xs = sSym[lane][kk], xa = sAsym[lane][kk];
-        const int es = wave_max(exp_of(xs)), ea = wave_max(exp_of(xa));
-        const long long g0 = (((long long)sl * nchf + c) * 2) * ncomp + cq;  // symmetric
-        const long long g1 = g0 + ncomp;                                       // antisymmetric
-        const int off = frag_off(kk, lane);
-        put_digits<S>(dig + (g0 * S * ntile + kt) * 1024, sstride, off, fix_of<S>(xs, es));
-        put_digits<S>(dig + (g1 * S * ntile + kt) * 1024, sstride, off, fix_of<S>(xa, ea));
-        if (lane == 0) {
-          dexp[g0 * nr_p + kt * 16 + kk] = es;
-          dexp[g1 * nr_p + kt * 16 + kk] = ea;
-        }
+      for (int u = 0; u < kFsGroup * 4; u++) {
+        const int f = u / 4, idx = tid + kSliceThreads * (u % 4);
+        const int kk = k_fast ? (idx & 15) : (idx >> 6);
+        const int tt = k_fast ? (idx >> 4) : (idx & 63);
+        const int t = c * 64 + tt;
+        const double w = t < nth_h ? weight[t] : 0.0;
+        sC[f][0][tt][kk] = (vn[u] + vs[u]) * w;
+        sC[f][1][tt][kk] = t != t_eq ? (vn[u] - vs[u]) * w : 0.0;
       }
       __syncthreads();
+      // wave task = (component, sym/asym, radius)
+      for (int task = wave; task < ng * 2 * 16; task += kSliceThreads / 64) {
+        const int kk = task & 15, h = (task >> 4) & 1, f = task >> 5;
+        const double x = sC[f][h][lane][kk];
+        const int e = wave_max(exp_of(x));
+        const long long g = (((long long)sl * nchf + c) * 2 + h) * ncomp + cg + f;
+        put_digits<S>(dig + (g * S * ntile + kt) * 1024, sstride, frag_off(kk, lane),
+                      fix_of<S>(x, e));
+        if (lane == 0) dexp[g * nr_p + kt * 16 + kk] = e;
+      }
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// GEMM side.  Work-groups of 8 waves; the data-operand blocks of a step are
-// copied global -> LDS (one 16-byte load per lane per 1 KiB block, LDS
-// address = block base + 16 lane) into one of two buffers while the other is
-// reduced.
+// GEMM side.  Work-groups of 8 waves.  Per step the shared operands (data
+// slices, the backward's P slices, and all exponents) are copied global ->
+// LDS without passing through registers (global_load_lds: lane L's 16 or 4
+// bytes land at block base + 16 L or 4 L) into one of two buffers while the
+// other is reduced; one s_waitcnt + barrier per step.
 constexpr int kGemmThreads = 512;
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) void gbl_void_t;
 
-__device__ __forceinline__ void copy_block(const signed char *src_lane, i4 *dst_block) {
+__device__ __forceinline__ void copy_block(const void *src_lane, void *dst_block) {
   __builtin_amdgcn_global_load_lds((gbl_void_t *)src_lane, (lds_void_t *)dst_block, 16, 0, 0);
+}
+__device__ __forceinline__ void copy_dword(const void *src_lane, void *dst_block) {
+  __builtin_amdgcn_global_load_lds((gbl_void_t *)src_lane, (lds_void_t *)dst_block, 4, 0, 0);
 }
 __device__ __forceinline__ void wait_copies() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
@@ -340,22 +380,74 @@ struct SplitDims {
   int L, nr, nr_p, nth, nth_h, nth_hp, ncomp, n_rt, ncg, nblk, nchf;
 };
 
+// Relative scale of group pair (g0, g0+1) (or of the single last group g0)
+// against the first pair: the pair value 256 acc_g0 + acc_(g0+1) carries
+// 2^(eA + eB + 4 - 8(g0+3)), the first pair 2^(eA + eB - 20).
+__device__ __forceinline__ double pair_scale(int g0, bool two) {
+  const int e = two ? -8 * g0 : 8 - 8 * g0;
+  return __longlong_as_double((long long)(1023 + e) << 52);
+}
+
+// Triangular sliced products of up to two quantities QA, QB (NQB of them)
+// with A slices AOF(q, s) and data slices BOF(q, t); T[j][r] collects
+// sum over pairs of pair_value * pair_scale, exactly as INT32 -> FP64.
+#define KSHT_SLICED_PRODUCTS(QA, QB, NQB, T)                                             \
+  {                                                                                    \
+    _Pragma("unroll") for (int j_ = 0; j_ < 2; j_++)                                   \
+      _Pragma("unroll") for (int r_ = 0; r_ < 4; r_++) T[j_][r_] = 0.0;                \
+    _Pragma("unroll") for (int g0 = 0; g0 < S; g0 += 2) {                              \
+      const bool two = g0 + 1 < S;                                                     \
+      i4 a0[2], a1[2];                                                                 \
+      _Pragma("unroll") for (int j_ = 0; j_ < 2; j_++) {                               \
+        a0[j_] = i4{0, 0, 0, 0};                                                       \
+        a1[j_] = i4{0, 0, 0, 0};                                                       \
+      }                                                                                \
+      _Pragma("unroll") for (int t = 0; t < S; t++) {                                  \
+        if (t > g0 + 1) continue;                                                      \
+        const int s0 = g0 - t, s1 = g0 + 1 - t;                                        \
+        _Pragma("unroll") for (int j_ = 0; j_ < NQB; j_++) {                           \
+          const int q = j_ ? (QB) : (QA);                                              \
+          const i4 b = BOF(q, t);                                                      \
+          if (s0 >= 0) a0[j_] = mfma8(AOF(q, s0), b, a0[j_]);                          \
+          if (two && s1 >= 0) a1[j_] = mfma8(AOF(q, s1), b, a1[j_]);                   \
+        }                                                                              \
+      }                                                                                \
+      const double sc = pair_scale(g0, two);                                           \
+      _Pragma("unroll") for (int j_ = 0; j_ < NQB; j_++)                               \
+        _Pragma("unroll") for (int r_ = 0; r_ < 4; r_++) {                             \
+          const int v = two ? a0[j_][r_] * 256 + a1[j_][r_] : a0[j_][r_];              \
+          T[j_][r_] = fma((double)v, sc, T[j_][r_]);                                   \
+        }                                                                              \
+    }                                                                                  \
+  }
+
 // ---------------------------------------------------------------------------
 // Backward.  Work-group = (M, radius tile, component group, theta block of
 // 64 rows); wave w owns theta tile w%4 for output slot w/4 (slot 0: m = +M,
-// slot 1: m = -M), over the steps (chunk c, parity) of both parities.  The
-// FP64 accumulators hold the north / south rows directly:
+// slot 1: m = -M), over the steps (chunk c, parity) of both parities.  LDS
+// per step: the data slices of both slots, the P slices of the four theta
+// tiles (fragment order) and the exponents; dP slices go straight to
+// registers, loaded at the start of the step and first used after the P
+// quantities.  The FP64 accumulators hold the north / south rows directly:
 //   even l-M: P terms add to both, dP terms (antisymmetric) to N and -S;
 //   odd l-M:  P terms N and -S, dP terms both,
 // and per output slot: vr = a^2 [P pol g3], vtheta = a ([dP dpol] - asin m' [P tor']),
 // vphi = a (-[dP tor] - asin m' [P dpol']), with primes on the conjugate slot.
+template <int S, int NB>
+struct BwdLds {
+  static constexpr int NBB = 2 * NB * S;  // data blocks [slot][component][digit]
+  static constexpr int NPB = 4 * S;       // P blocks [theta tile][digit]
+  static constexpr int I4 = (NBB + NPB) * 64 + 64;  // + 256 exponents
+  static constexpr int bytes() { return 2 * I4 * 16; }
+};
+
 template <int S, int NB, bool VEC>
 __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
     SplitDims d, int nd_first, SplitTabs tb, const long long *__restrict__ cb,
     const signed char *__restrict__ dig, const int *__restrict__ dexp, double *__restrict__ out,
     GridAddr ga, const double *__restrict__ a_r, const double *__restrict__ asin_t) {
-  extern __shared__ i4 smem[];  // [2 buffers][2 slots][NB][S][64 lanes]
-  constexpr int NBLK = 2 * NB * S;
+  extern __shared__ i4 smem[];
+  using LY = BwdLds<S, NB>;
   constexpr int NQ = VEC ? 5 : NB;
   constexpr int NO = VEC ? 3 : NB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -368,7 +460,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
   const int M = (int)(bidx / d.n_rt);
   const int L = d.L;
   const int n = L - M + 1, np0 = (n + 1) / 2, np1 = n / 2;
-  const int nch0 = (np0 + 63) / 64, nch1 = (np1 + 63) / 64;
+  const int nch0 = bchunk_count(M, np0), nch1 = bchunk_count(M, np1);
   const int nsig = M == 0 ? 1 : 2;
   const int tt = wave & 3, sg = wave >> 2;
   const int t0 = (tbk * 4 + tt) * 16;
@@ -377,20 +469,39 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
   const int ntile = d.nr_p >> 4;
   const int sgc = nsig == 1 ? 0 : 1 - sg;  // conjugate slot
   const int sl_own = sg == 0 ? L + M : L - M;
-  const int sl_oth = sgc == 0 ? L + M : L - M;
   const int kcol = rt * 16 + (lane & 15);
 
-  auto issue = [&](int st) {
-    const int c = st >> 1, par = st & 1;
-    i4 *buf = smem + (st & 1) * NBLK * 64;
-    for (int b = wave; b < NBLK; b += kGemmThreads / 64) {
-      const int sgb = b / (NB * S), f = (b / S) % NB, s = b % S;
+  auto issue = [&](int c, int par, int b) {
+    i4 *buf = smem + b * LY::I4;
+    for (int blk = wave; blk < LY::NBB; blk += kGemmThreads / 64) {
+      const int sgb = blk / (NB * S), f = (blk / S) % NB, s = blk % S;
       const int sl = sgb == 0 ? L + M : L - M;
       const long long gb = (cb[2 * sl + par] + c) * d.ncomp + nd0 + f;
-      copy_block(dig + ((gb * S + s) * ntile + rt) * 1024 + lane * 16, buf + b * 64);
+      copy_block(dig + ((gb * S + s) * ntile + rt) * 1024 + lane * 16, buf + blk * 64);
+    }
+    const long long so = tb.off[2 * M + par] + (long long)c * S * d.nth_hp * 64;
+    for (int blk = wave; blk < LY::NPB; blk += kGemmThreads / 64) {
+      const int t2 = blk / S, s = blk % S;
+      const signed char *src = tb.P + so +
+                               ((long long)s * d.nth_hp + (tbk * 4 + t2) * 16 + (lane & 15)) * 64 +
+                               16 * (lane >> 4);
+      copy_block(src, buf + (LY::NBB + blk) * 64);
+    }
+    int *ex = reinterpret_cast<int *>(buf + (LY::NBB + LY::NPB) * 64);
+    const long long eo = tb.eoff[2 * M + par] + (long long)c * d.nth_hp + tbk * 64 + lane;
+    if (wave == 0) copy_dword(tb.eP + eo, ex);
+    if (wave == 1 && VEC) copy_dword(tb.edP + eo, ex + 64);
+    if (wave == 2 || wave == 3) {
+      const int sgf = (wave - 2) * 4 + (lane >> 4);
+      const int *src = dexp;
+      if (sgf < 2 * NB) {
+        const int sgb = sgf / NB, f = sgf % NB;
+        const int sl = sgb == 0 ? L + M : L - M;
+        src = dexp + ((cb[2 * sl + par] + c) * d.ncomp + nd0 + f) * d.nr_p + rt * 16 + (lane & 15);
+      }
+      copy_dword(src, ex + 128 + (wave - 2) * 64);
     }
   };
-  auto step_ok = [&](int st) { return st < 2 * nch0 && !((st & 1) && (st >> 1) >= nch1); };
 
   double C[NO][2][4];
 #pragma unroll
@@ -409,51 +520,38 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
     }
   }
 
-  issue(0);
-  for (int st = 0; step_ok(st); st++) {
-    wait_copies();
-    __syncthreads();
-    if (step_ok(st + 1)) issue(st + 1);
-    if (!active) continue;
-    const int c = st >> 1, par = st & 1;
-    const i4 *buf = smem + (st & 1) * NBLK * 64;
-    // P / dP slices of this wave's theta tile
-    const long long ab = tb.off[2 * M + par] +
-                         ((long long)c * S * d.nth_hp + t0 + (lane & 15)) * 64 + 16 * (lane >> 4);
-    i4 aP[S], adP[S];
+  issue(0, 0, 0);
+  int st = 0;
+  for (int c = 0; c < nch0; c++) {
 #pragma unroll
-    for (int s = 0; s < S; s++) {
-      aP[s] = *reinterpret_cast<const i4 *>(tb.P + ab + (long long)s * d.nth_hp * 64);
-      if (VEC) adP[s] = *reinterpret_cast<const i4 *>(tb.dP + ab + (long long)s * d.nth_hp * 64);
-    }
-    const long long eo = tb.eoff[2 * M + par] + (long long)c * d.nth_hp + t0 + 4 * (lane >> 4);
-    const long long go = (cb[2 * sl_own + par] + c) * d.ncomp + nd0;
-    const long long gc = (cb[2 * sl_oth + par] + c) * d.ncomp + nd0;
-    int ex[NQ][4];
-    if (VEC) {
-      const int ebp = dexp[(go + 0) * d.nr_p + kcol], ebd = dexp[(go + 1) * d.nr_p + kcol];
-      const int ebt = dexp[(go + 2) * d.nr_p + kcol];
-      const int ebtc = dexp[(gc + 2) * d.nr_p + kcol], ebdc = dexp[(gc + 1) * d.nr_p + kcol];
+    for (int par = 0; par < 2; par++) {
+      if (par == 1 && c >= nch1) continue;
+      wait_copies();
+      __syncthreads();
+      // next step: (c, 1), else (c + 1, 0)
+      if (par == 0 && c < nch1)
+        issue(c, 1, (st + 1) & 1);
+      else if (c + 1 < nch0)
+        issue(c + 1, 0, (st + 1) & 1);
+      if (active) {
+        const i4 *buf = smem + (st & 1) * LY::I4;
+        const int *ex = reinterpret_cast<const int *>(buf + (LY::NBB + LY::NPB) * 64);
+        i4 aP[S], adP[S];
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int eP = tb.eP[eo + r], eD = tb.edP[eo + r];
-        ex[0][r] = eP + ebp;
-        ex[1 % NQ][r] = eP + ebtc;
-        ex[2 % NQ][r] = eP + ebdc;
-        ex[3 % NQ][r] = eD + ebd;
-        ex[4 % NQ][r] = eD + ebt;
-      }
-    } else {
+        for (int s = 0; s < S; s++) aP[s] = buf[(LY::NBB + tt * S + s) * 64 + lane];
+        if (VEC) {
+          const long long ab = tb.off[2 * M + par] +
+                               ((long long)c * S * d.nth_hp + t0 + (lane & 15)) * 64 + 16 * (lane >> 4);
 #pragma unroll
-      for (int q = 0; q < NQ; q++) {
-        const int eb = dexp[(go + q) * d.nr_p + kcol];
-#pragma unroll
-        for (int r = 0; r < 4; r++) ex[q][r] = tb.eP[eo + r] + eb;
-      }
-    }
-    const i4 *bo = buf + sg * NB * S * 64 + lane;
-    const i4 *bc = buf + sgc * NB * S * 64 + lane;
-    // quantity q: A table, data block, output, N / S coefficients
+          for (int s = 0; s < S; s++)
+            adP[s] = *reinterpret_cast<const i4 *>(tb.dP + ab + (long long)s * d.nth_hp * 64);
+        }
+        const int rb = tt * 16 + 4 * (lane >> 4);
+        const int lk = lane & 15;
+        auto ecol = [&](int sgb, int f) { return ex[128 + (sgb * NB + f) * 16 + lk]; };
+        const i4 *bo = buf + sg * NB * S * 64 + lane;
+        const i4 *bc = buf + sgc * NB * S * 64 + lane;
+        // quantity q: A table, data block
 #define AOF(q, s) ((VEC && (q) >= 3) ? adP[s] : aP[s])
 #define BOF(q, t)                                                            \
   (VEC ? ((q) == 0   ? bo[(0 * S + (t)) * 64]                                \
@@ -462,64 +560,52 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
           : (q) == 3 ? bo[(1 * S + (t)) * 64]                                \
                      : bo[(2 * S + (t)) * 64])                               \
        : bo[((q) * S + (t)) * 64])
-    // quantities two at a time: 4 independent INT32 accumulator chains
-#pragma unroll
-    for (int qb = 0; qb < NQ; qb += 2) {
-      __builtin_amdgcn_sched_barrier(0);  // keep the register footprint to two quantities
-#pragma unroll
-      for (int g0 = 0; g0 < S; g0 += 2) {
-        const bool two = g0 + 1 < S;
-        i4 a0[2], a1[2];
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-          a0[j] = i4{0, 0, 0, 0};
-          a1[j] = i4{0, 0, 0, 0};
-        }
-#pragma unroll
-        for (int t = 0; t < S; t++) {
-          if (t > g0 + 1) continue;
-          const int s0 = g0 - t, s1 = g0 + 1 - t;
-#pragma unroll
-          for (int j = 0; j < 2; j++) {
-            const int q = qb + j;
-            if (q >= NQ) continue;
-            const i4 b = BOF(q, t);
-            if (s0 >= 0) a0[j] = mfma8(AOF(q, s0), b, a0[j]);
-            if (two && s1 >= 0) a1[j] = mfma8(AOF(q, s1), b, a1[j]);
-          }
-        }
-        const int sh = two ? 4 - 8 * (g0 + 3) : 4 - 8 * (g0 + 2);
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-          const int q = qb + j;
-          if (q >= NQ) continue;
+        // fold of quantity q with the collected T: x = ldexp(T, eA + eB - 20)
+        auto fold = [&](int q, const double (&T)[4]) {
+          const int eb = VEC ? (q == 0 ? ecol(sg, 0) : q == 1 ? ecol(sgc, 2) : q == 2 ? ecol(sgc, 1)
+                                : q == 3 ? ecol(sg, 1) : ecol(sg, 2))
+                             : ecol(sg, q);
           const int o = VEC ? (q == 0 ? 0 : (q == 1 || q == 3) ? 1 : 2) : q;
 #pragma unroll
           for (int r = 0; r < 4; r++) {
-            const int v = two ? a0[j][r] * 256 + a1[j][r] : a0[j][r];
-            const double x = ldexp((double)v, ex[q][r] + sh);
-            double cn, cs;
-            if (!VEC || q == 0) {
-              cn = 1.0;
-              cs = par ? -1.0 : 1.0;
-            } else if (q <= 2) {
-              cn = fr[r];
-              cs = par ? -fr[r] : fr[r];
-            } else if (q == 3) {
-              cn = 1.0;
-              cs = par ? 1.0 : -1.0;
+            const int ea = ex[(VEC && q >= 3 ? 64 : 0) + rb + r];
+            const double x = ldexp(T[r], ea + eb - 20);
+            // P terms even in theta about the equator for even l-M, dP terms odd
+            const bool sym = (!VEC || q <= 2) ? (par == 0) : (par == 1);
+            if (!VEC || q == 0 || q == 3) {
+              C[o][0][r] += x;
+              C[o][1][r] += sym ? x : -x;
+            } else if (q == 4) {
+              C[o][0][r] -= x;
+              C[o][1][r] -= sym ? x : -x;
             } else {
-              cn = -1.0;
-              cs = par ? -1.0 : 1.0;
+              C[o][0][r] = fma(x, fr[r], C[o][0][r]);
+              C[o][1][r] = fma(x, sym ? fr[r] : -fr[r], C[o][1][r]);
             }
-            C[o][0][r] = fma(x, cn, C[o][0][r]);
-            C[o][1][r] = fma(x, cs, C[o][1][r]);
           }
+        };
+        double T[2][4];
+        // P quantities first (dP slices arrive meanwhile), then dP quantities
+        __builtin_amdgcn_sched_barrier(0);
+        KSHT_SLICED_PRODUCTS(0, 1 % NQ, (NQ > 1 ? 2 : 1), T)
+        fold(0, T[0]);
+        if (NQ > 1) fold(1 % NQ, T[1]);
+        if (NQ > 2) {
+          __builtin_amdgcn_sched_barrier(0);
+          KSHT_SLICED_PRODUCTS(2 % NQ, 2 % NQ, 1, T)
+          fold(2 % NQ, T[0]);
         }
-      }
-    }
+        if (VEC) {
+          __builtin_amdgcn_sched_barrier(0);
+          KSHT_SLICED_PRODUCTS(3 % NQ, 4 % NQ, 2, T)
+          fold(3 % NQ, T[0]);
+          fold(4 % NQ, T[1]);
+        }
 #undef AOF
 #undef BOF
+      }
+      st++;
+    }
   }
   if (!active) return;
   const int k = kcol;
@@ -536,13 +622,13 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
     const long long os = base + (long long)(d.nth - 1 - t) * ga.s_t;
 #pragma unroll
     for (int o = 0; o < NO; o++) {
-      const double sc = VEC ? (o == 0 ? a2 : a1) : 1.0;
+      const double scl = VEC ? (o == 0 ? a2 : a1) : 1.0;
       double *dst = out + (long long)(nd0 + o) * ga.s_nd;
       if (eq) {
-        dst[on] = sc * (0.5 * (C[o][0][r] + C[o][1][r]));
+        dst[on] = scl * (0.5 * (C[o][0][r] + C[o][1][r]));
       } else {
-        dst[on] = sc * C[o][0][r];
-        dst[os] = sc * C[o][1][r];
+        dst[on] = scl * C[o][0][r];
+        dst[os] = scl * C[o][1][r];
       }
     }
   }
@@ -557,14 +643,26 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
 // conjugate slot):
 //   pol = r^2 g7 [P vr], dpol = r g7 ([dP vt] - m [P/sin vp']),
 //   tor = r g7 (-[dP vp] - m [P/sin vt']); scalars g6 [P s].
+// LDS per step: the data slices of both slots and the exponents; the wave's
+// own P, dP, P/sin slices go to registers, P prefetched one step ahead and
+// dP, P/sin loaded while the preceding quantities reduce.
+template <int S, int NB>
+struct FwdLds {
+  static constexpr int NBB = 4 * NB * S;  // [slot][sym/asym][component][digit]
+  static constexpr int NEX = 64 * NB + 3 * 128;  // column + row exponents
+  static constexpr int I4 = NBB * 64 + (NEX + 3) / 4;
+  static constexpr int NBUF = (2 * (I4 * 16) <= 160 * 1024) ? 2 : 1;
+  static constexpr int bytes() { return NBUF * I4 * 16; }
+};
+
 template <int S, int NB, bool VEC>
 __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
     SplitDims d, int nd_first, SplitTabs tb, const signed char *__restrict__ dig,
     const int *__restrict__ dexp, double *__restrict__ sp, SpecAddr sa,
     const int *__restrict__ jst, const int *__restrict__ ipad, const double *__restrict__ radius) {
-  extern __shared__ i4 smem[];  // [buffers][2 slots][sym/asym][NB][S][64 lanes]
-  constexpr int NBLK = 4 * NB * S;
-  constexpr int NBUF = (2 * NBLK <= 160) ? 2 : 1;
+  extern __shared__ i4 smem[];
+  using LY = FwdLds<S, NB>;
+  constexpr int NBUF = LY::NBUF;
   constexpr int NQ = VEC ? 5 : NB;
   constexpr int NO = VEC ? 3 : NB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -589,12 +687,32 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
   const int nchf = d.nchf;
 
   auto issue = [&](int c) {
-    i4 *buf = smem + (c % NBUF) * NBLK * 64;
-    for (int b = wave; b < NBLK; b += kGemmThreads / 64) {
+    i4 *buf = smem + (c % NBUF) * LY::I4;
+    for (int b = wave; b < LY::NBB; b += kGemmThreads / 64) {
       const int sgb = b / (2 * NB * S), h = (b / (NB * S)) & 1, f = (b / S) % NB, s = b % S;
       const int sl = sgb == 0 ? L + M : L - M;
       const long long gb = (((long long)sl * nchf + c) * 2 + h) * d.ncomp + nd0 + f;
       copy_block(dig + ((gb * S + s) * ntile + rt) * 1024 + lane * 16, buf + b * 64);
+    }
+    int *ex = reinterpret_cast<int *>(buf + LY::NBB * 64);
+    // column exponents [slot][sym/asym][component][16 radii] (four groups per
+    // instruction) and row exponents [P, dP, P/sin][parity][64 rows]
+    const int ntask = NB + (VEC ? 6 : 2);
+    for (int task = wave; task < ntask; task += kGemmThreads / 64) {
+      if (task < NB) {
+        const int g = task * 4 + (lane >> 4);
+        const int sgb = g / (2 * NB), h = (g / NB) & 1, f = g % NB;
+        const int sl = sgb == 0 ? L + M : L - M;
+        const long long gb = (((long long)sl * nchf + c) * 2 + h) * d.ncomp + nd0 + f;
+        copy_dword(dexp + gb * d.nr_p + rt * 16 + (lane & 15), ex + task * 64);
+      } else {
+        const int tw = task - NB, tab = tw >> 1, pr = tw & 1;
+        const int ipp = pr ? ip1 : ip0;
+        const int row = ib * 64 + lane;
+        const int *et = tab == 0 ? tb.eP : tab == 1 ? tb.edP : tb.ePa;
+        const int *src = row < ipp ? et + tb.eoff[2 * M + pr] + (long long)c * ipp + row : et;
+        copy_dword(src, ex + 64 * NB + tw * 64);
+      }
     }
   };
 
@@ -607,100 +725,79 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
       for (int r = 0; r < 4; r++) C[a][o][r] = 0.0;
 
   const int hx = par, hy = 1 - par;  // P: symmetric for even l-M; dP the other one
+  const long long abase = tb.off[2 * M + par] + ((long long)i0 + (lane & 15)) * 64 + 16 * (lane >> 4);
+  auto load_a = [&](const signed char *tab, int c, i4 (&a)[S]) {
+#pragma unroll
+    for (int s = 0; s < S; s++)
+      a[s] = *reinterpret_cast<const i4 *>(tab + abase + ((long long)c * S + s) * ip * 64);
+  };
+  i4 aP[S], adP[S], aPa[S];
   issue(0);
   for (int c = 0; c < nchf; c++) {
     wait_copies();
     __syncthreads();
     if (NBUF == 2 && c + 1 < nchf) issue(c + 1);
     if (active) {
-      const i4 *buf = smem + (c % NBUF) * NBLK * 64;
-      const long long ab = tb.off[2 * M + par] + ((long long)c * S * ip + i0 + (lane & 15)) * 64 +
-                           16 * (lane >> 4);
-      auto load_a = [&](const signed char *tab, i4 (&a)[S]) {
-#pragma unroll
-        for (int s = 0; s < S; s++)
-          a[s] = *reinterpret_cast<const i4 *>(tab + ab + (long long)s * ip * 64);
-      };
-      const long long eo = tb.eoff[2 * M + par] + (long long)c * ip + i0 + 4 * (lane >> 4);
-      int eP[4], eD[4], eA[4];
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        eP[r] = tb.eP[eo + r];
-        eD[r] = VEC ? tb.edP[eo + r] : 0;
-        eA[r] = VEC ? tb.ePa[eo + r] : 0;
-      }
+      const i4 *buf = smem + (c % NBUF) * LY::I4;
+      const int *ex = reinterpret_cast<const int *>(buf + LY::NBB * 64);
+      load_a(tb.P, c, aP);
+      if (VEC) load_a(tb.dP, c, adP);
       const int sgc0 = nsig == 1 ? 0 : 1;  // conjugate of slot 0 (slot 1's is 0)
-      auto eb = [&](int sgb, int h, int f) {
-        const int sl = sgb == 0 ? L + M : L - M;
-        return dexp[((((long long)sl * nchf + c) * 2 + h) * d.ncomp + nd0 + f) * d.nr_p + kcol];
-      };
+      const int rb = 64 * NB + par * 64 + (wave & 3) * 16 + 4 * (lane >> 4);
+      const int lk = lane & 15;
       // quantity q of slot sg: data block (slot, sym/asym, component)
 #define QSLOT(q, sg) ((VEC && ((q) == 2 || (q) == 4)) ? ((sg) == 0 ? sgc0 : 0) : (sg))
 #define QH(q) ((!VEC || (q) == 0 || (q) == 2 || (q) == 4) ? hx : hy)
 #define QF(q) (!VEC ? (q) : (q) == 0 ? 0 : ((q) == 1 || (q) == 4) ? 1 : 2)
-#define BOF(q, sg, t) \
-  buf[(((QSLOT(q, sg) * 2 + QH(q)) * NB + QF(q)) * S + (t)) * 64 + lane]
-      // one quantity at a time for both slots (4 INT32 accumulator chains);
-      // AA = the A slices of its table, EA = their row exponents.  M = 0 has
-      // one slot: slot 1 then repeats slot 0 (same data) and is not stored.
-#define KSHT_FWD_QUANTITY(Q, AA, EA)                                                         \
+#define BOFS(q, sg, t) buf[(((QSLOT(q, sg) * 2 + QH(q)) * NB + QF(q)) * S + (t)) * 64 + lane]
+      // one quantity for both slots: the two slots play the role of the two
+      // quantities of KSHT_SLICED_PRODUCTS; M = 0 has one slot (slot 1 then
+      // repeats slot 0 and is not stored)
+#define KSHT_FWD_QUANTITY(Q, TAB)                                                           \
   {                                                                                         \
-    constexpr int q = (Q);                                                                  \
+    constexpr int qq = (Q);                                                                 \
     __builtin_amdgcn_sched_barrier(0);                                                      \
-    int exq[2][4];                                                                          \
+    double T[2][4];                                                                         \
+    KSHT_SLICED_PRODUCTS(0, 1, 2, T)                                                        \
+    constexpr int o = VEC ? (qq == 0 ? 0 : (qq <= 2) ? 1 : 2) : qq;                         \
     _Pragma("unroll") for (int sg = 0; sg < 2; sg++) {                                      \
-      const int e = eb(QSLOT(q, sg), QH(q), QF(q));                                         \
-      _Pragma("unroll") for (int r = 0; r < 4; r++) exq[sg][r] = EA[r] + e;                 \
-    }                                                                                       \
-    constexpr int o = VEC ? (q == 0 ? 0 : (q <= 2) ? 1 : 2) : q;                            \
-    _Pragma("unroll") for (int g0 = 0; g0 < S; g0 += 2) {                                   \
-      const bool two = g0 + 1 < S;                                                          \
-      i4 a0[2], a1[2];                                                                      \
-      _Pragma("unroll") for (int j = 0; j < 2; j++) {                                       \
-        a0[j] = i4{0, 0, 0, 0};                                                             \
-        a1[j] = i4{0, 0, 0, 0};                                                             \
-      }                                                                                     \
-      _Pragma("unroll") for (int t = 0; t < S; t++) {                                       \
-        if (t > g0 + 1) continue;                                                           \
-        const int s0 = g0 - t, s1 = g0 + 1 - t;                                             \
-        _Pragma("unroll") for (int sg = 0; sg < 2; sg++) {                                  \
-          const i4 b = BOF(q, sg, t);                                                       \
-          if (s0 >= 0) a0[sg] = mfma8(AA[s0], b, a0[sg]);                                   \
-          if (two && s1 >= 0) a1[sg] = mfma8(AA[s1], b, a1[sg]);                            \
-        }                                                                                   \
-      }                                                                                     \
-      const int sh = two ? 4 - 8 * (g0 + 3) : 4 - 8 * (g0 + 2);                             \
-      _Pragma("unroll") for (int sg = 0; sg < 2; sg++) {                                    \
-        const double ms = sg == 0 ? (double)M : -(double)M;                                 \
-        const double cf = (!VEC || q <= 1) ? 1.0 : (q == 3 ? -1.0 : -ms);                   \
-        _Pragma("unroll") for (int r = 0; r < 4; r++) {                                     \
-          const int v = two ? a0[sg][r] * 256 + a1[sg][r] : a0[sg][r];                      \
-          C[sg][o][r] = fma(ldexp((double)v, exq[sg][r] + sh), cf, C[sg][o][r]);            \
-        }                                                                                   \
+      const double ms = sg == 0 ? (double)M : -(double)M;                                   \
+      const double cf = (!VEC || qq <= 1) ? 1.0 : (qq == 3 ? -1.0 : -ms);                   \
+      const int eb = ex[((QSLOT(qq, sg) * 2 + QH(qq)) * NB + QF(qq)) * 16 + lk];            \
+      _Pragma("unroll") for (int r = 0; r < 4; r++) {                                       \
+        const int ea = ex[rb + (TAB) * 128 + r];                                            \
+        C[sg][o][r] = fma(ldexp(T[sg][r], ea + eb - 20), cf, C[sg][o][r]);                  \
       }                                                                                     \
     }                                                                                       \
   }
-      // table order P, dP, P/sin, each loaded just before its quantities
-      i4 aP[S], adP[S], aPa[S];
-      load_a(tb.P, aP);
+#define AOF(q, s) AA_[s]
+#define BOF(q, t) BOFS(qq, (q), t)
+#define AA_ aP
+      KSHT_FWD_QUANTITY(0, 0)
+#undef AA_
       if (VEC) {
-        load_a(tb.dP, adP);
-        KSHT_FWD_QUANTITY(0, aP, eP)
-        load_a(tb.Pa, aPa);
-        KSHT_FWD_QUANTITY(1 % NQ, adP, eD)
-        KSHT_FWD_QUANTITY(3 % NQ, adP, eD)
-        KSHT_FWD_QUANTITY(2 % NQ, aPa, eA)
-        KSHT_FWD_QUANTITY(4 % NQ, aPa, eA)
+        load_a(tb.Pa, c, aPa);
+#define AA_ adP
+        KSHT_FWD_QUANTITY(1 % NQ, 1)
+        KSHT_FWD_QUANTITY(3 % NQ, 1)
+#undef AA_
+#define AA_ aPa
+        KSHT_FWD_QUANTITY(2 % NQ, 2)
+        KSHT_FWD_QUANTITY(4 % NQ, 2)
+#undef AA_
       } else {
-        KSHT_FWD_QUANTITY(0, aP, eP)
-        if (NQ > 1) KSHT_FWD_QUANTITY(1 % NQ, aP, eP)
-        if (NQ > 2) KSHT_FWD_QUANTITY(2 % NQ, aP, eP)
+#define AA_ aP
+        if (NQ > 1) KSHT_FWD_QUANTITY(1 % NQ, 0)
+        if (NQ > 2) KSHT_FWD_QUANTITY(2 % NQ, 0)
+#undef AA_
       }
 #undef KSHT_FWD_QUANTITY
+#undef AOF
+#undef BOF
+#undef BOFS
 #undef QSLOT
 #undef QH
 #undef QF
-#undef BOF
     }
     if (NBUF == 1 && c + 1 < nchf) {
       __syncthreads();
@@ -733,6 +830,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
     }
   }
 }
+#undef KSHT_SLICED_PRODUCTS
 
 // ---------------------------------------------------------------------------
 // Host side.
@@ -748,7 +846,7 @@ static int table_layouts(sht_plan *p, SplitState *st) {
     const int n = L - M + 1;
     for (int par = 0; par < 2; par++) {
       const int np = par ? n / 2 : (n + 1) / 2;
-      const int nchb = (np + 63) / 64;
+      const int nchb = bchunk_count(M, np);
       const int ip = p->h_ipad[2 * M + par];
       st->hb_off[2 * M + par] = bo;
       st->hb_eoff[2 * M + par] = be;
@@ -768,7 +866,7 @@ static int table_layouts(sht_plan *p, SplitState *st) {
     const int M = sl >= L ? sl - L : L - sl, n = L - M + 1;
     for (int par = 0; par < 2; par++) {
       st->hcb[2 * sl + par] = ch;
-      ch += ((par ? n / 2 : (n + 1) / 2) + 63) / 64;
+      ch += bchunk_count(M, par ? n / 2 : (n + 1) / 2);
     }
   }
   st->nchb = ch;
@@ -807,16 +905,18 @@ static int table_layouts(sht_plan *p, SplitState *st) {
 }
 
 template <int S>
-static constexpr int bwd_lds(int nb) { return 2 * 2 * nb * S * 1024; }
+static constexpr int bwd_lds(int nb) {
+  return nb == 3 ? BwdLds<S, 3>::bytes() : nb == 2 ? BwdLds<S, 2>::bytes() : BwdLds<S, 1>::bytes();
+}
 template <int S>
 static constexpr int fwd_lds(int nb) {
-  return ((2 * 4 * nb * S <= 160) ? 2 : 1) * 4 * nb * S * 1024;
+  return nb == 3 ? FwdLds<S, 3>::bytes() : nb == 2 ? FwdLds<S, 2>::bytes() : FwdLds<S, 1>::bytes();
 }
 
 template <int S>
 static int slice_tables(sht_plan *p, SplitState *st) {
   const int L = p->L;
-  const int nchb_max = ((L + 2) / 2 + 63) / 64;
+  const int nchb_max = bchunk_count(0, (L + 2) / 2);
   const dim3 gb((p->nth_hp + 255) / 256, nchb_max, 2 * (L + 1));
   hipLaunchKernelGGL(k_slice_b<S>, gb, dim3(256), 0, p->stream, L, p->nth_hp, p->t.offb,
                      p->t.ipad, p->t.Pb, st->b_off, st->b_eoff, st->bP, st->beP);
@@ -914,7 +1014,8 @@ static int bwd_split_S(sht_plan *p, int nv, int ns, const double *spec, SpecAddr
                        GridAddr ga) {
   SplitState *st = (SplitState *)p->split;
   const int ncomp = 3 * nv + ns;
-  hipLaunchKernelGGL(k_bslice<S>, dim3(p->nr_p, p->nslot), dim3(kSliceThreads), 0, p->stream,
+  hipLaunchKernelGGL(k_bslice<S>, dim3(p->nr_p, p->nslot), dim3(kSliceThreads),
+                     sizeof(double) * 128 * ncomp, p->stream,
                      p->L, p->nr, p->nr_p, nv, ncomp, spec, sa, p->t.jst, st->cb, st->dig,
                      st->dexp);
   KSHT_HIP(hipGetLastError());
