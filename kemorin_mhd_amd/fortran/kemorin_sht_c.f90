@@ -1,0 +1,210 @@
+!> @file  kemorin_sht_c.f90
+!!       ISO_C_BINDING interface to libkemorin_sht.so (include/kemorin_sht.h)
+!!
+!! Host-side binding for the Fortran solver: the plan descriptor as an
+!! interoperable derived type and one interface per C entry point.  Status
+!! codes are returned as integer(c_int) (0 = ok); sht_get_last_error gives
+!! the message.  Device buffers are passed as type(c_ptr) (hipMalloc'd or
+!! mapped), host buffers of the *_host entry points as assumed-size arrays.
+      module kemorin_sht_c
+!
+      use iso_c_binding
+      implicit none
+!
+      integer(c_int), parameter :: SHT_OK = 0
+      integer(c_int), parameter :: SHT_PREC_FP64 = 0
+      integer(c_int), parameter :: SHT_PREC_SPLIT_INT8 = 1
+!
+!>      Plan descriptor (sht_desc)
+      type, bind(C) :: sht_desc
+        integer(c_int) :: l_truncation
+        integer(c_int) :: nidx_r
+        integer(c_int) :: nidx_theta
+        integer(c_int) :: nidx_phi
+        type(c_ptr) :: radius
+        integer(c_int) :: max_ncomp
+        integer(c_int) :: precision
+        integer(c_int) :: n_slices
+        integer(c_int) :: flags
+      end type sht_desc
+!
+      interface
+!
+        integer(c_int) function sht_plan_create(desc, plan)           &
+     &                          bind(C, name = 'sht_plan_create')
+          import :: c_int, c_ptr, sht_desc
+          type(sht_desc), intent(in) :: desc
+          type(c_ptr), intent(out) :: plan
+        end function sht_plan_create
+!
+        integer(c_int) function sht_plan_destroy(plan)                 &
+     &                          bind(C, name = 'sht_plan_destroy')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: plan
+        end function sht_plan_destroy
+!
+        integer(c_int) function sht_plan_set_stream(plan, stream)      &
+     &                          bind(C, name = 'sht_plan_set_stream')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: plan, stream
+        end function sht_plan_set_stream
+!
+        integer(c_int) function sht_plan_get_table(plan, name, out, n) &
+     &                          bind(C, name = 'sht_plan_get_table')
+          import :: c_int, c_ptr, c_char, c_long_long
+          type(c_ptr), value :: plan
+          character(kind = c_char), intent(in) :: name(*)
+          type(c_ptr), value :: out
+          integer(c_long_long), value :: n
+        end function sht_plan_get_table
+!
+        integer(c_long_long) function sht_plan_device_bytes(plan)      &
+     &                          bind(C, name = 'sht_plan_device_bytes')
+          import :: c_long_long, c_ptr
+          type(c_ptr), value :: plan
+        end function sht_plan_device_bytes
+!
+        integer(c_int) function sht_plan_timer_enable(plan, on)        &
+     &                          bind(C, name = 'sht_plan_timer_enable')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: plan
+          integer(c_int), value :: on
+        end function sht_plan_timer_enable
+!
+        integer(c_int) function sht_plan_timer_read(plan, ms, calls, n) &
+     &                          bind(C, name = 'sht_plan_timer_read')
+          import :: c_int, c_ptr, c_double, c_long_long
+          type(c_ptr), value :: plan
+          real(c_double), intent(out) :: ms(*)
+          integer(c_long_long), intent(out) :: calls(*)
+          integer(c_int), value :: n
+        end function sht_plan_timer_read
+!
+!>      sph_backward_transforms (SS/t_sph_transforms.f90:74-123), device
+        integer(c_int) function sht_backward(plan, nvector, nscalar,   &
+     &                          d_WS_rj, d_v_rtp)                      &
+     &                          bind(C, name = 'sht_backward')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: plan
+          integer(c_int), value :: nvector, nscalar
+          type(c_ptr), value :: d_WS_rj, d_v_rtp
+        end function sht_backward
+!
+!>      sph_forward_transforms (SS/t_sph_transforms.f90:250-297), device
+        integer(c_int) function sht_forward(plan, nvector, nscalar,    &
+     &                          d_v_rtp, d_WR_rj)                      &
+     &                          bind(C, name = 'sht_forward')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: plan
+          integer(c_int), value :: nvector, nscalar
+          type(c_ptr), value :: d_v_rtp, d_WR_rj
+        end function sht_forward
+!
+!>      Same with host arrays (one H2D and one D2H copy)
+        integer(c_int) function sht_backward_host(plan, nvector,       &
+     &                          nscalar, WS_rj, v_rtp)                 &
+     &                          bind(C, name = 'sht_backward_host')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: plan
+          integer(c_int), value :: nvector, nscalar
+          real(c_double), intent(in) :: WS_rj(*)
+          real(c_double), intent(out) :: v_rtp(*)
+        end function sht_backward_host
+!
+        integer(c_int) function sht_forward_host(plan, nvector,        &
+     &                          nscalar, v_rtp, WR_rj)                 &
+     &                          bind(C, name = 'sht_forward_host')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: plan
+          integer(c_int), value :: nvector, nscalar
+          real(c_double), intent(in) :: v_rtp(*)
+          real(c_double), intent(out) :: WR_rj(*)
+        end function sht_forward_host
+!
+!>      Stage entry points (sel_[backward|forward]_legendre_trans,
+!!      back_FFT_select_from_recv / fwd_FFT_select_to_send,
+!!      calypso_sph_comm_N), device buffers in the reference layouts
+        integer(c_int) function sht_legendre_backward(plan, nvector,   &
+     &                          nscalar, d_WR_rlm, d_WS_rtm)           &
+     &                          bind(C, name = 'sht_legendre_backward')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: plan
+          integer(c_int), value :: nvector, nscalar
+          type(c_ptr), value :: d_WR_rlm, d_WS_rtm
+        end function sht_legendre_backward
+!
+        integer(c_int) function sht_legendre_forward(plan, nvector,    &
+     &                          nscalar, d_WR_rtm, d_WS_rlm)           &
+     &                          bind(C, name = 'sht_legendre_forward')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: plan
+          integer(c_int), value :: nvector, nscalar
+          type(c_ptr), value :: d_WR_rtm, d_WS_rlm
+        end function sht_legendre_forward
+!
+        integer(c_int) function sht_fft_backward(plan, ncomp,          &
+     &                          d_WR_rtp, d_v_rtp)                     &
+     &                          bind(C, name = 'sht_fft_backward')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: plan
+          integer(c_int), value :: ncomp
+          type(c_ptr), value :: d_WR_rtp, d_v_rtp
+        end function sht_fft_backward
+!
+        integer(c_int) function sht_fft_forward(plan, ncomp,           &
+     &                          d_v_rtp, d_WS_rtp)                     &
+     &                          bind(C, name = 'sht_fft_forward')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: plan
+          integer(c_int), value :: ncomp
+          type(c_ptr), value :: d_v_rtp, d_WS_rtp
+        end function sht_fft_forward
+!
+        integer(c_int) function sht_exchange_rtm_to_rtp(plan, ncomp,   &
+     &                          d_rtm, d_rtp)                          &
+     &                          bind(C, name = 'sht_exchange_rtm_to_rtp')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: plan
+          integer(c_int), value :: ncomp
+          type(c_ptr), value :: d_rtm, d_rtp
+        end function sht_exchange_rtm_to_rtp
+!
+        integer(c_int) function sht_exchange_rtp_to_rtm(plan, ncomp,   &
+     &                          d_rtp, d_rtm)                          &
+     &                          bind(C, name = 'sht_exchange_rtp_to_rtm')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: plan
+          integer(c_int), value :: ncomp
+          type(c_ptr), value :: d_rtp, d_rtm
+        end function sht_exchange_rtp_to_rtm
+!
+        integer(c_int) function sht_exchange_gather_rev(plan, ncomp,   &
+     &                          n_dst, n_src, d_irev, d_src, d_dst)    &
+     &                          bind(C, name = 'sht_exchange_gather_rev')
+          import :: c_int, c_ptr, c_long_long
+          type(c_ptr), value :: plan
+          integer(c_int), value :: ncomp
+          integer(c_long_long), value :: n_dst, n_src
+          type(c_ptr), value :: d_irev, d_src, d_dst
+        end function sht_exchange_gather_rev
+!
+        integer(c_int) function sht_selftest()                         &
+     &                          bind(C, name = 'sht_selftest')
+          import :: c_int
+        end function sht_selftest
+!
+        integer(c_int) function sht_get_last_error(buf, n)             &
+     &                          bind(C, name = 'sht_get_last_error')
+          import :: c_int, c_char
+          character(kind = c_char), intent(out) :: buf(*)
+          integer(c_int), value :: n
+        end function sht_get_last_error
+!
+        type(c_ptr) function sht_version()                             &
+     &                          bind(C, name = 'sht_version')
+          import :: c_ptr
+        end function sht_version
+!
+      end interface
+!
+      end module kemorin_sht_c

@@ -115,3 +115,45 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stdout
+
+
+AMDFLANG = "/opt/rocm/bin/amdflang"
+
+
+@pytest.mark.skipif(not os.path.exists(AMDFLANG), reason="no Fortran compiler")
+def test_fortran_binding_links_and_validates(tmp_path):
+    """The ISO_C_BINDING module compiles, links against the library and a
+    Fortran caller gets SHT_ERR_ARG for a bad descriptor (no device used)."""
+    mod = os.path.join(ROOT, "kemorin_mhd_amd", "fortran", "kemorin_sht_c.f90")
+    prog = tmp_path / "drv.f90"
+    prog.write_text("""
+      program drv
+      use iso_c_binding
+      use kemorin_sht_c
+      implicit none
+      type(sht_desc) :: d
+      type(c_ptr) :: plan
+      real(c_double), target :: r(4)
+      integer(c_int) :: rc
+      r = (/0.5d0, 0.8d0, 1.1d0, 1.5d0/)
+      d%l_truncation = 0
+      d%nidx_r = 4
+      d%nidx_theta = 12
+      d%nidx_phi = 16
+      d%radius = c_loc(r)
+      d%max_ncomp = 9
+      d%precision = SHT_PREC_FP64
+      d%n_slices = 0
+      d%flags = 0
+      rc = sht_plan_create(d, plan)
+      write(*, '(a,i0)') 'rc=', rc
+      end program drv
+""")
+    exe = tmp_path / "drv"
+    libdir = os.path.dirname(sht.LIB_PATH)
+    r = subprocess.run([AMDFLANG, "-J", str(tmp_path), mod, str(prog), "-o", str(exe),
+                        f"-L{libdir}", "-lkemorin_sht", f"-Wl,-rpath,{libdir}"],
+                       capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert "rc=1" in out.stdout, out.stdout + out.stderr
