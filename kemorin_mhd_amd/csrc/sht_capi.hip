@@ -163,9 +163,9 @@ extern "C" int sht_fft_backward(sht_plan *p, int ncomp, const double *WR_rtp, do
 extern "C" int sht_fft_forward(sht_plan *p, int ncomp, const double *v_rtp, double *WS_rtp) {
   int rc = check_call(p, 0, ncomp, v_rtp, WS_rtp, "sht_fft_forward");
   if (rc) return rc;
-  const size_t bytes = sizeof(double) * (size_t)ncomp * p->nr * p->nth * p->nphi;
   StageTimer tm(p, 2);
-  KSHT_HIP(hipMemsetAsync(WS_rtp, 0, bytes, p->stream));
+  rc = launch_zero_unused_rtp(p, ncomp, WS_rtp);
+  if (rc) return rc;
   return launch_fft_fwd(p, ncomp, v_rtp, WS_rtp, rtp_fourier_addr(p, ncomp));
 }
 

@@ -310,8 +310,8 @@ __device__ __forceinline__ void f3_passes(double2 (&v)[8], double2 *ln, int j,
       const int k = b % Ns;
 #pragma unroll
       for (int r = 1; r < R; r++) {
-        // exp(-2 pi i k r / (Ns R)) = twn[k r (NP / (Ns R)) R0]
-        double2 w = twn[(k * r * (NP / (Ns * R)) * R0) % N];
+        // exp(-2 pi i k r / (Ns R)) = twn[k r (NP / (Ns R)) R0]; k r < Ns R, so < N
+        double2 w = twn[k * r * (NP / (Ns * R)) * R0];
         w.y *= -s;
         v[i * R + r] = cmul(v[i * R + r], w);
       }
@@ -351,6 +351,14 @@ __device__ __forceinline__ constexpr int f3_pos(int q) {
   return (q % (8 / R)) * R + q / (8 / R);
 }
 
+// twiddle indices of the R0 aliases: m j1 and (m + N - NP) j1 modulo N, for
+// m < NP, j1 < R0 (N = R0 NP): the first is below N, the second equals
+// m j1 + N - NP j1 for j1 >= 1
+__device__ __forceinline__ int alias_lo(int m, int j1) { return m * j1; }
+__device__ __forceinline__ int alias_hi(int m, int j1, int N, int NP) {
+  return j1 == 0 ? 0 : m * j1 + N - NP * j1;
+}
+
 __device__ __forceinline__ double2 cfma(double2 a, double2 w, double2 acc) {
   return make_double2(acc.x + a.x * w.x - a.y * w.y, acc.y + a.x * w.y + a.y * w.x);
 }
@@ -364,7 +372,7 @@ __global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long
   constexpr int T = NP / 8;
   const int tid = threadIdx.x, cl = tid / T, j = tid % T;
   const int nd = blockIdx.y;
-  const long long line0 = (long long)xcd_block(blockIdx.x, gridDim.x) * 16;
+  const long long line0 = (long long)blockIdx.x * 16;
   double *stage = reinterpret_cast<double *>(lds);
   double2 zlo[8], zhi[8];
 #pragma unroll
@@ -374,11 +382,23 @@ __global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long
   for (int f0 = 0; f0 <= L; f0 += 256) {
     const int f1 = min(f0 + 256, L + 1);
     const int q0 = f0 == 0 ? 0 : 2 * f0 - 1, nrow = 2 * (f1 - 1) - q0 + 1;
-    for (int idx = tid; idx < nrow * 16; idx += NP) {
-      const int row = idx >> 4, l16 = idx & 15;
-      const long long line = line0 + l16;
-      stage[row * kStageStride + l16] =
-          line < nline ? X[xa.q_off[q0 + row] * xa.q_mul + xbase + line * xa.s_line] : 0.0;
+    // batches of 8 independent loads per thread, then the LDS stores
+    for (int base = 0; base < nrow * 16; base += 8 * NP) {
+      double val[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int idx = base + tid + u * NP;
+        const int row = idx >> 4, l16 = idx & 15;
+        const long long line = line0 + l16;
+        val[u] = (idx < nrow * 16 && line < nline)
+                     ? X[xa.q_off[q0 + row] * xa.q_mul + xbase + line * xa.s_line]
+                     : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int idx = base + tid + u * NP;
+        if (idx < nrow * 16) stage[(idx >> 4) * kStageStride + (idx & 15)] = val[u];
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -410,8 +430,8 @@ __global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const int m = j + T * q;
-      double2 t1 = twn[(int)(((long long)m * j1) % N)];
-      double2 t2 = twn[(int)(((long long)(m + N - NP) * j1) % N)];
+      double2 t1 = twn[alias_lo(m, j1)];
+      double2 t2 = twn[alias_hi(m, j1, N, NP)];
       t1.y = -t1.y;  // e^{+...}
       t2.y = -t2.y;
       w[q] = cfma(zhi[q], t2, cmul(zlo[q], t1));
@@ -450,7 +470,7 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
   constexpr int T = NP / 8;
   const int tid = threadIdx.x, cl = tid / T, j = tid % T;
   const int nd = blockIdx.y;
-  const long long line0 = (long long)xcd_block(blockIdx.x, gridDim.x) * 16;
+  const long long line0 = (long long)blockIdx.x * 16;
   const long long nnod = nline * N;
   const bool vec2 = (nline & 1) == 0;
   double2 zlo[8], zhi[8];
@@ -458,17 +478,25 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
   for (int q = 0; q < 8; q++) zlo[q] = zhi[q] = make_double2(0.0, 0.0);
   double2 *ln = lds + cl * (NP + NP / 8);
   for (int j1 = 0; j1 < R0; j1++) {
-    for (int idx = tid; idx < NP * 8; idx += NP) {
-      const int j2 = idx >> 3, c = idx & 7;
-      const long long la = line0 + 2 * c;
-      const double *src = v + (long long)nd * nnod + (long long)(R0 * j2 + j1) * nline + la;
-      double2 z = make_double2(0.0, 0.0);
-      if (la + 1 < nline) {
-        z = vec2 ? *reinterpret_cast<const double2 *>(src) : make_double2(src[0], src[1]);
-      } else if (la < nline) {
-        z.x = src[0];
+    {
+      double2 z[8];  // eight independent loads per thread, then the LDS stores
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int idx = tid + u * NP, j2 = idx >> 3, c = idx & 7;
+        const long long la = line0 + 2 * c;
+        const double *src = v + (long long)nd * nnod + (long long)(R0 * j2 + j1) * nline + la;
+        z[u] = make_double2(0.0, 0.0);
+        if (la + 1 < nline) {
+          z[u] = vec2 ? *reinterpret_cast<const double2 *>(src) : make_double2(src[0], src[1]);
+        } else if (la < nline) {
+          z[u].x = src[0];
+        }
       }
-      lds[j2 * 9 + c] = z;
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int idx = tid + u * NP;
+        lds[(idx >> 3) * 9 + (idx & 7)] = z[u];
+      }
     }
     __syncthreads();
     double2 w[8];
@@ -480,9 +508,8 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
     for (int q = 0; q < 8; q++) {
       const int k = j + T * q;
       const double2 y = w[f3_pos<NP>(q)];
-      if (k <= L) zlo[q] = cfma(y, twn[(int)(((long long)k * j1) % N)], zlo[q]);
-      if (k >= NP - L)
-        zhi[q] = cfma(y, twn[(int)(((long long)(k + N - NP) * j1) % N)], zhi[q]);
+      if (k <= L) zlo[q] = cfma(y, twn[alias_lo(k, j1)], zlo[q]);
+      if (k >= NP - L) zhi[q] = cfma(y, twn[alias_hi(k, j1, N, NP)], zhi[q]);
     }
   }
   // Z_{N-m} for m = j + T q sits at index NP - m of the lane (T - j) % T of

@@ -141,6 +141,7 @@ int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa,
 int launch_fft_fwd(sht_plan *p, int ncomp, const double *v, double *X,
                    FourierAddr xa);
 // reorders (sht_reorder.hip)
+int launch_zero_unused_rtp(sht_plan *p, int ncomp, double *rtp);
 int launch_rtm_to_rtp(sht_plan *p, int ncomp, const double *rtm, double *rtp);
 int launch_rtp_to_rtm(sht_plan *p, int ncomp, const double *rtp, double *rtm);
 int launch_gather_rev(sht_plan *p, int ncomp, long long n_dst, long long n_src,

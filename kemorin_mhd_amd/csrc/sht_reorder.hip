@@ -8,6 +8,8 @@
 // The permutations move whole ncomp-vectors; tiles of 16 x 16 nodes go
 // through LDS so that both the t-contiguous rtm side and the k-contiguous
 // rtp side are read / written in coalesced rows.
+#include <algorithm>
+
 #include "sht_internal.h"
 
 namespace ksht {
@@ -68,9 +70,29 @@ __global__ void __launch_bounds__(256) k_gather_rev(int ncomp, long long n_dst, 
   dst[gid] = (r >= 1 && r <= n_src) ? src[nd + (long long)ncomp * (r - 1)] : 0.0;
 }
 
+__global__ void __launch_bounds__(256) k_fill_zero(double *__restrict__ a, long long n) {
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    a[i] = 0.0;
+}
+
+// Zero the FFTPACK slots of an rtp buffer that carry no order m <= L: the
+// Nyquist slot p = 1 and p >= 2L+2 (p is the slowest index).  A fill kernel,
+// not hipMemsetAsync, so that captured graphs replay it faithfully.
+int launch_zero_unused_rtp(sht_plan *p, int ncomp, double *rtp) {
+  const long long slot = (long long)ncomp * p->nr * p->nth;
+  const long long tail = slot * (p->nphi - 2 * p->L - 2);
+  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::min<long long>((slot + 255) / 256, 4096)),
+                     dim3(256), 0, p->stream, rtp + slot, slot);
+  if (tail > 0)
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::min<long long>((tail + 255) / 256, 65536)),
+                       dim3(256), 0, p->stream, rtp + slot * (2 * p->L + 2), tail);
+  KSHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+
 int launch_rtm_to_rtp(sht_plan *p, int ncomp, const double *rtm, double *rtp) {
-  const size_t bytes = sizeof(double) * (size_t)ncomp * p->nr * p->nth * p->nphi;
-  KSHT_HIP(hipMemsetAsync(rtp, 0, bytes, p->stream));
+  int rc = launch_zero_unused_rtp(p, ncomp, rtp);
+  if (rc) return rc;
   const dim3 grid((p->nth + 15) / 16, (p->nr + 15) / 16, p->nslot);
   hipLaunchKernelGGL(k_rtm_rtp<0>, grid, dim3(256), 0, p->stream, p->L, p->nr, p->nth, ncomp, rtm,
                      rtp);

@@ -270,9 +270,21 @@ __global__ void __launch_bounds__(kSliceThreads) k_bslice(
     const int i0 = bchunk_start(M, c), ni = bchunk_end(M, c) - i0;
     const int d0 = 2 * i0, nd_run = min(2 * ni, n - d0) * ncomp;
     __syncthreads();
-    for (int idx = tid; idx < nd_run; idx += kSliceThreads) {
-      const int dd = idx / ncomp, cq = idx - dd * ncomp;
-      run[idx] = k < nr ? src[(long long)(d0 + dd) * sa.s_j + (long long)cq * sa.s_nd] : 0.0;
+    for (int base = 0; base < nd_run; base += 8 * kSliceThreads) {
+      double val[8];  // independent loads first, then the LDS stores
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int idx = base + tid + u * kSliceThreads;
+        const int dd = idx / ncomp, cq = idx - dd * ncomp;
+        val[u] = (idx < nd_run && k < nr)
+                     ? src[(long long)(d0 + dd) * sa.s_j + (long long)cq * sa.s_nd]
+                     : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int idx = base + tid + u * kSliceThreads;
+        if (idx < nd_run) run[idx] = val[u];
+      }
     }
     __syncthreads();
     const int ntask = (c < nch1 ? 2 : 1) * ncomp;
