@@ -42,8 +42,20 @@ def build(verbose: bool = False) -> str:
             print(r.stderr)
         return obj
 
-    with ThreadPoolExecutor(max_workers=min(6, os.cpu_count() or 1)) as ex:
+    def waits():
+        # the split GEMMs count their own memory waits: verify the generated
+        # code keeps every in-flight load's registers untouched on all paths
+        from .isa_waits import check_source
+        probs = check_source(cc, FLAGS, os.path.join(CSRC, "sht_split.hip"), "k_leg_")
+        bad = {k: v for k, v in probs.items() if v}
+        if not probs or bad:
+            raise RuntimeError("split GEMM wait check failed:\n" +
+                               "\n".join(f"{k}: {v[:5]}" for k, v in bad.items()))
+
+    with ThreadPoolExecutor(max_workers=min(7, os.cpu_count() or 1)) as ex:
+        fut = ex.submit(waits)
         objs = list(ex.map(one, SOURCES))
+        fut.result()
     tmp = LIB + ".tmp"
     r = subprocess.run([cc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs],
                        capture_output=True, text=True)

@@ -17,11 +17,11 @@
 // ldexp((double)pair, eA + eB + 4 - 8(g+3)).  FP64 remains on the GPU for P
 // generation, Gauss weights, the operand exponents and this recombination.
 //
-// Slices of the data operand are stored in MFMA-fragment order: a 1 KiB block
-// holds 16 columns x 64 K bytes with lane L's 16 bytes (column L%16,
-// K = 16 (L/16) .. +15) at offset 16 L, so a wave moves a block with one
-// 16-byte load per lane, straight from global memory into LDS.  P-table
-// slices are row-major 64-byte rows (row r, K bytes 0..63).
+// Slices are stored as 1 KiB blocks of 16 rows (P tables) or 16 columns
+// (data operand) x 64 K bytes, one 64-byte row each; a wave moves a block
+// straight from global memory into LDS with one 16-byte load per lane, lane L
+// taking row L%16, K bytes 16 (L/16) .. +15, which lands the block in MFMA
+// fragment order (lane L's bytes at 16 L).
 //
 // INT8 MFMA layout (sht_selftest): A(16x64) lane holds row lane%16, bytes
 // K = 16*(lane/16) .. +15; B(64x16) lane holds column lane%16, the same K
@@ -228,10 +228,12 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-// byte offset of (column c16, K index q) in a fragment-order block
-__device__ __forceinline__ int frag_off(int c16, int q) {
-  return (((q >> 4) << 4) + c16) * 16 + (q & 15);
-}
+// byte offset of (column c16, K index q) in a 1 KiB slice block: row-major,
+// one 64-byte row per column, so a slicing wave (lane = K index) writes whole
+// rows; the GEMM copies a block into LDS in MFMA fragment order by giving
+// lane L the source bytes (row L%16, K 16 (L/16) .. +15)
+__device__ __forceinline__ int slice_off(int c16, int q) { return c16 * 64 + q; }
+__device__ __forceinline__ int frag_src(int lane) { return (lane & 15) * 64 + 16 * (lane >> 4); }
 
 template <int S>
 __device__ __forceinline__ void put_digits(signed char *blk, long long sstride, int off,
@@ -263,7 +265,7 @@ __global__ void __launch_bounds__(kSliceThreads) k_bslice(
   const int n = L - M + 1, np0 = (n + 1) / 2, np1 = n / 2;
   const int nch0 = bchunk_count(M, np0), nch1 = bchunk_count(M, np1);
   const int ntile = nr_p >> 4;
-  const int off = frag_off(k & 15, lane);
+  const int off = slice_off(k & 15, lane);
   const long long sstride = (long long)ntile * 1024;
   const double *src = sp + (long long)jst[sl] * sa.s_j + (long long)k * sa.s_k;
   for (int c = 0; c < nch0; c++) {
@@ -361,7 +363,7 @@ __global__ void __launch_bounds__(kSliceThreads) k_fslice(
         const double x = sC[f][h][lane][kk];
         const int e = wave_max(exp_of(x));
         const long long g = (((long long)sl * nchf + c) * 2 + h) * ncomp + cg + f;
-        put_digits<S>(dig + (g * S * ntile + kt) * 1024, sstride, frag_off(kk, lane),
+        put_digits<S>(dig + (g * S * ntile + kt) * 1024, sstride, slice_off(kk, lane),
                       fix_of<S>(x, e));
         if (lane == 0) dexp[g * nr_p + kt * 16 + kk] = e;
       }
@@ -374,19 +376,62 @@ __global__ void __launch_bounds__(kSliceThreads) k_fslice(
 // slices, the backward's P slices, and all exponents) are copied global ->
 // LDS without passing through registers (global_load_lds: lane L's 16 or 4
 // bytes land at block base + 16 L or 4 L) into one of two buffers while the
-// other is reduced; one s_waitcnt + barrier per step.
+// other is reduced.  These copies and the A-slice register loads are issued
+// as inline assembly, outside the compiler's s_waitcnt bookkeeping: the
+// compiler cannot tell that a copy writes the other buffer and would wait for
+// it before every LDS read.  The kernels count their vector-memory operations
+// themselves (issue order = completion order for vmcnt): one wait for
+// everything at the top of a step, and partial waits that leave only the
+// next step's copies in flight before the A slices of this step are used.
 constexpr int kGemmThreads = 512;
 
-typedef __attribute__((address_space(3))) void lds_void_t;
-typedef __attribute__((address_space(1))) void gbl_void_t;
-
-__device__ __forceinline__ void copy_block(const void *src_lane, void *dst_block) {
-  __builtin_amdgcn_global_load_lds((gbl_void_t *)src_lane, (lds_void_t *)dst_block, 16, 0, 0);
+__device__ __forceinline__ unsigned lds_addr(const void *p) {
+  return (unsigned)(size_t)(const __attribute__((address_space(3))) void *)p;
 }
-__device__ __forceinline__ void copy_dword(const void *src_lane, void *dst_block) {
-  __builtin_amdgcn_global_load_lds((gbl_void_t *)src_lane, (lds_void_t *)dst_block, 4, 0, 0);
+// wave-uniform base (SGPRs) + 32-bit lane offset, as the hardware addresses
+// them: keeps the address arithmetic scalar and one VGPR per lane offset
+__device__ __forceinline__ const void *uniform_ptr(const void *p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (const void *)(((unsigned long long)hi << 32) | lo);
 }
-__device__ __forceinline__ void wait_copies() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// dst: LDS byte address of the destination (lane i lands at dst + 16 i or
+// dst + 4 i), computed from the work-group's LDS base as an integer.
+// The compiler's hazard recognizer does not look inside inline assembly, so
+// the statements provide the wait states themselves: an LDS DMA needs one
+// after an SALU write of M0 (which the compiler places right before the
+// statement); a vector-memory instruction needs five after a VALU write
+// (v_readfirstlane) of an SGPR it reads, which the exponent copies' runtime-
+// selected bases get.  The build checks the generated code for both.
+__device__ __forceinline__ void copy_block(const void *base, unsigned voff, unsigned dst) {
+  asm volatile("s_nop 0\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(voff), "s"(uniform_ptr(base)),
+               "{m0}"(__builtin_amdgcn_readfirstlane(dst))
+               : "memory");
+}
+__device__ __forceinline__ void copy_dword(const void *base, unsigned voff, unsigned dst) {
+  asm volatile("s_nop 4\n\tglobal_load_lds_dword %0, %1" ::"v"(voff), "s"(uniform_ptr(base)),
+               "{m0}"(__builtin_amdgcn_readfirstlane(dst))
+               : "memory");
+}
+__device__ __forceinline__ i4 load_slice(const void *base, unsigned voff) {
+  i4 v;
+  asm volatile("global_load_dwordx4 %0, %1, %2" : "=v"(v) : "v"(voff), "s"(uniform_ptr(base)) : "memory");
+  return v;
+}
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+__device__ __forceinline__ void wait_copies() { wait_vm<0>(); }
+// registers loaded by load_slice are used only after a wait_vm that covers
+// them; this ties each use to the wait
+template <int S>
+__device__ __forceinline__ void landed(i4 (&a)[S]) {
+#pragma unroll
+  for (int s = 0; s < S; s++) asm volatile("" : "+v"(a[s]));
+}
 
 struct SplitDims {
   int L, nr, nr_p, nth, nth_h, nth_hp, ncomp, n_rt, ncg, nblk, nchf;
@@ -449,7 +494,11 @@ template <int S, int NB>
 struct BwdLds {
   static constexpr int NBB = 2 * NB * S;  // data blocks [slot][component][digit]
   static constexpr int NPB = 4 * S;       // P blocks [theta tile][digit]
-  static constexpr int I4 = (NBB + NPB) * 64 + 64;  // + 256 exponents
+  // every wave issues the same number of copies (spare ones into dummy
+  // space), so the compiler can count them in s_waitcnt vmcnt
+  static constexpr int KB = (NBB + NPB + 7) / 8;
+  static constexpr int EX = 8 * KB * 64;  // i4 offset of the exponents
+  static constexpr int I4 = EX + 64 + 64;  // 256 exponents + 256 dummy
   static constexpr int bytes() { return 2 * I4 * 16; }
 };
 
@@ -462,7 +511,8 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
   using LY = BwdLds<S, NB>;
   constexpr int NQ = VEC ? 5 : NB;
   constexpr int NO = VEC ? 3 : NB;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform per wave
   long long bidx = blockIdx.x;
   const int tbk = (int)(bidx % d.nblk);
   bidx /= d.nblk;
@@ -483,36 +533,65 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
   const int sl_own = sg == 0 ? L + M : L - M;
   const int kcol = rt * 16 + (lane & 15);
 
+  // chunk bases of the data slices of both slots and parities (the
+  // block and row indices below fit in 32 bits)
+  const int cb00 = (int)cb[2 * (L + M)], cb01 = (int)cb[2 * (L + M) + 1];
+  const int cb10 = (int)cb[2 * (L - M)], cb11 = (int)cb[2 * (L - M) + 1];
+  // chunk base of slot sgb (0/1) and parity par, as arithmetic (nested
+  // selects of four values compile to a table in scratch)
+  auto cbase = [&](int sgb, int par) {
+    const int c0 = par ? cb01 : cb00, c1 = par ? cb11 : cb10;
+    return c0 + sgb * (c1 - c0);
+  };
+  // P-slice 64-byte row and exponent index of (M, parity), chunk 0 (scalars:
+  // a runtime-indexed array would live in scratch)
+  const int prow0 = (int)(tb.off[2 * M] >> 6), prow1 = (int)(tb.off[2 * M + 1] >> 6);
+  const int erow0 = (int)tb.eoff[2 * M], erow1 = (int)tb.eoff[2 * M + 1];
+  constexpr int KCOPY = LY::KB + 1;  // copies per wave per step
+  const unsigned fsrc = (unsigned)frag_src(lane);
+  const unsigned lbase = lds_addr(smem);
   auto issue = [&](int c, int par, int b) {
-    i4 *buf = smem + b * LY::I4;
-    for (int blk = wave; blk < LY::NBB; blk += kGemmThreads / 64) {
-      const int sgb = blk / (NB * S), f = (blk / S) % NB, s = blk % S;
-      const int sl = sgb == 0 ? L + M : L - M;
-      const long long gb = (cb[2 * sl + par] + c) * d.ncomp + nd0 + f;
-      copy_block(dig + ((gb * S + s) * ntile + rt) * 1024 + lane * 16, buf + blk * 64);
+    const unsigned bufa = lbase + 16u * (unsigned)(b * LY::I4);
+    const int pr0 = (par ? prow1 : prow0) + c * S * d.nth_hp + tbk * 64;
+#pragma unroll
+    for (int j = 0; j < LY::KB; j++) {
+      const int blk = wave + 8 * j;
+      const signed char *base;
+      if (blk < LY::NBB) {
+        const int sgb = blk / (NB * S), f = (blk / S) % NB, s = blk % S;
+        const int gb = (cbase(sgb, par) + c) * d.ncomp + nd0 + f;
+        base = dig + (size_t)((gb * S + s) * ntile + rt) * 1024;
+      } else if (blk < LY::NBB + LY::NPB) {
+        const int t2 = (blk - LY::NBB) / S, s = (blk - LY::NBB) % S;
+        base = tb.P + (size_t)(pr0 + s * d.nth_hp + t2 * 16) * 64;
+      } else {
+        base = tb.P + (size_t)pr0 * 64;  // spare copy
+      }
+      copy_block(base, fsrc, bufa + 1024u * blk);
     }
-    const long long so = tb.off[2 * M + par] + (long long)c * S * d.nth_hp * 64;
-    for (int blk = wave; blk < LY::NPB; blk += kGemmThreads / 64) {
-      const int t2 = blk / S, s = blk % S;
-      const signed char *src = tb.P + so +
-                               ((long long)s * d.nth_hp + (tbk * 4 + t2) * 16 + (lane & 15)) * 64 +
-                               16 * (lane >> 4);
-      copy_block(src, buf + (LY::NBB + blk) * 64);
-    }
-    int *ex = reinterpret_cast<int *>(buf + (LY::NBB + LY::NPB) * 64);
-    const long long eo = tb.eoff[2 * M + par] + (long long)c * d.nth_hp + tbk * 64 + lane;
-    if (wave == 0) copy_dword(tb.eP + eo, ex);
-    if (wave == 1 && VEC) copy_dword(tb.edP + eo, ex + 64);
-    if (wave == 2 || wave == 3) {
+    const unsigned exa = bufa + 16u * LY::EX;  // exponents, 4 bytes each
+    const int eo = (par ? erow1 : erow0) + c * d.nth_hp + tbk * 64;
+    const int *ebase = tb.eP + eo;
+    unsigned eoff = 4u * lane;
+    unsigned edst = exa + 4u * (256 + (wave & 3) * 64);  // spare copies
+    if (wave == 0) {
+      edst = exa;
+    } else if (wave == 1) {
+      if (VEC) ebase = tb.edP + eo;
+      edst = exa + 4u * 64;
+    } else if (wave < 4) {
+      // four (slot, component) groups of 16 radii per instruction
       const int sgf = (wave - 2) * 4 + (lane >> 4);
-      const int *src = dexp;
+      ebase = dexp;
+      eoff = 0;
       if (sgf < 2 * NB) {
         const int sgb = sgf / NB, f = sgf % NB;
-        const int sl = sgb == 0 ? L + M : L - M;
-        src = dexp + ((cb[2 * sl + par] + c) * d.ncomp + nd0 + f) * d.nr_p + rt * 16 + (lane & 15);
+        eoff = 4u * (unsigned)(((cbase(sgb, par) + c) * d.ncomp + nd0 + f) * d.nr_p + rt * 16 +
+                               (lane & 15));
       }
-      copy_dword(src, ex + 128 + (wave - 2) * 64);
+      edst = exa + 4u * (128 + (wave - 2) * 64);
     }
+    copy_dword(ebase, eoff, edst);
   };
 
   double C[NO][2][4];
@@ -540,23 +619,27 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
       if (par == 1 && c >= nch1) continue;
       wait_copies();
       __syncthreads();
-      // next step: (c, 1), else (c + 1, 0)
-      if (par == 0 && c < nch1)
-        issue(c, 1, (st + 1) & 1);
-      else if (c + 1 < nch0)
-        issue(c + 1, 0, (st + 1) & 1);
-      if (active) {
-        const i4 *buf = smem + (st & 1) * LY::I4;
-        const int *ex = reinterpret_cast<const int *>(buf + (LY::NBB + LY::NPB) * 64);
-        i4 aP[S], adP[S];
+      const i4 *buf = smem + (st & 1) * LY::I4;
+      const int *ex = reinterpret_cast<const int *>(buf + LY::EX);
+      // dP slices of this step first (register loads), then the next step's
+      // copies, so waiting for dP does not wait for the copies
+      // Every wave issues the same vector-memory operations on every path
+      // (rows t0 < nth_hp exist for idle waves too; the last step copies
+      // itself again into the idle buffer), so the counted waits below hold
+      // on all paths.
+      i4 aP[S], adP[S];
+      if (VEC) {
+        const int r0 = (par ? prow1 : prow0) + c * S * d.nth_hp + t0;
 #pragma unroll
-        for (int s = 0; s < S; s++) aP[s] = buf[(LY::NBB + tt * S + s) * 64 + lane];
-        if (VEC) {
-          const long long ab = tb.off[2 * M + par] +
-                               ((long long)c * S * d.nth_hp + t0 + (lane & 15)) * 64 + 16 * (lane >> 4);
+        for (int s = 0; s < S; s++) adP[s] = load_slice(tb.dP + (size_t)(r0 + s * d.nth_hp) * 64, fsrc);
+      }
+      // next step: (c, 1), else (c + 1, 0), else this one again
+      const bool nx1 = par == 0 && c < nch1, nx0 = !nx1 && c + 1 < nch0;
+      issue(nx0 ? c + 1 : c, nx1 ? 1 : nx0 ? 0 : par, (st + 1) & 1);
+      {
+        if (active) {
 #pragma unroll
-          for (int s = 0; s < S; s++)
-            adP[s] = *reinterpret_cast<const i4 *>(tb.dP + ab + (long long)s * d.nth_hp * 64);
+          for (int s = 0; s < S; s++) aP[s] = buf[(LY::NBB + tt * S + s) * 64 + lane];
         }
         const int rb = tt * 16 + 4 * (lane >> 4);
         const int lk = lane & 15;
@@ -598,20 +681,28 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
         };
         double T[2][4];
         // P quantities first (dP slices arrive meanwhile), then dP quantities
-        __builtin_amdgcn_sched_barrier(0);
-        KSHT_SLICED_PRODUCTS(0, 1 % NQ, (NQ > 1 ? 2 : 1), T)
-        fold(0, T[0]);
-        if (NQ > 1) fold(1 % NQ, T[1]);
-        if (NQ > 2) {
+        if (active) {
           __builtin_amdgcn_sched_barrier(0);
-          KSHT_SLICED_PRODUCTS(2 % NQ, 2 % NQ, 1, T)
-          fold(2 % NQ, T[0]);
+          KSHT_SLICED_PRODUCTS(0, 1 % NQ, (NQ > 1 ? 2 : 1), T)
+          fold(0, T[0]);
+          if (NQ > 1) fold(1 % NQ, T[1]);
+          if (NQ > 2) {
+            __builtin_amdgcn_sched_barrier(0);
+            KSHT_SLICED_PRODUCTS(2 % NQ, 2 % NQ, 1, T)
+            fold(2 % NQ, T[0]);
+          }
         }
         if (VEC) {
-          __builtin_amdgcn_sched_barrier(0);
-          KSHT_SLICED_PRODUCTS(3 % NQ, 4 % NQ, 2, T)
-          fold(3 % NQ, T[0]);
-          fold(4 % NQ, T[1]);
+          // on every path (idle waves too): the registers stay allocated
+          // to the loads until they have landed
+          wait_vm<KCOPY>();  // dP slices landed; the copies stay in flight
+          landed(adP);
+          if (active) {
+            __builtin_amdgcn_sched_barrier(0);
+            KSHT_SLICED_PRODUCTS(3 % NQ, 4 % NQ, 2, T)
+            fold(3 % NQ, T[0]);
+            fold(4 % NQ, T[1]);
+          }
         }
 #undef AOF
 #undef BOF
@@ -619,6 +710,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
       st++;
     }
   }
+  wait_vm<0>();
   if (!active) return;
   const int k = kcol;
   if (k >= d.nr) return;
@@ -661,8 +753,13 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
 template <int S, int NB>
 struct FwdLds {
   static constexpr int NBB = 4 * NB * S;  // [slot][sym/asym][component][digit]
-  static constexpr int NEX = 64 * NB + 3 * 128;  // column + row exponents
-  static constexpr int I4 = NBB * 64 + (NEX + 3) / 4;
+  // column exponents (NB tasks of 4 groups) and row exponents (2 or 6
+  // tasks); every wave issues KB block and KE exponent copies (spares into
+  // dummy space) so that s_waitcnt vmcnt can count them
+  static constexpr int NET = NB + 6;
+  static constexpr int KB = (NBB + 7) / 8, KE = (NET + 7) / 8;
+  static constexpr int EX = 8 * KB * 64;       // i4 offset of the exponents
+  static constexpr int I4 = EX + 8 * KE * 16;  // 64 ints per exponent task
   static constexpr int NBUF = (2 * (I4 * 16) <= 160 * 1024) ? 2 : 1;
   static constexpr int bytes() { return NBUF * I4 * 16; }
 };
@@ -677,7 +774,8 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
   constexpr int NBUF = LY::NBUF;
   constexpr int NQ = VEC ? 5 : NB;
   constexpr int NO = VEC ? 3 : NB;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform per wave
   long long bidx = blockIdx.x;
   const int ib = (int)(bidx % d.nblk);
   bidx /= d.nblk;
@@ -698,33 +796,44 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
   const int kcol = rt * 16 + (lane & 15);
   const int nchf = d.nchf;
 
-  auto issue = [&](int c) {
-    i4 *buf = smem + (c % NBUF) * LY::I4;
-    for (int b = wave; b < LY::NBB; b += kGemmThreads / 64) {
-      const int sgb = b / (2 * NB * S), h = (b / (NB * S)) & 1, f = (b / S) % NB, s = b % S;
-      const int sl = sgb == 0 ? L + M : L - M;
-      const long long gb = (((long long)sl * nchf + c) * 2 + h) * d.ncomp + nd0 + f;
-      copy_block(dig + ((gb * S + s) * ntile + rt) * 1024 + lane * 16, buf + b * 64);
+  // 32-bit block / row indices (they fit): slice group of (slot, chunk 0, sym/asym)
+  // is ((sl * nchf + c) * 2 + h) * ncomp + component
+  const int gsl[2] = {(L + M) * nchf * 2 * d.ncomp, (L - M) * nchf * 2 * d.ncomp};
+  const int erow0 = (int)tb.eoff[2 * M], erow1 = (int)tb.eoff[2 * M + 1];
+  const unsigned fsrc = (unsigned)frag_src(lane);
+  const unsigned lbase = lds_addr(smem);
+  auto issue = [&](int c, int b) {
+    const unsigned bufa = lbase + 16u * (unsigned)(b * LY::I4);
+#pragma unroll
+    for (int j = 0; j < LY::KB; j++) {
+      const int b = wave + 8 * j;
+      const int bb = b < LY::NBB ? b : 0;  // spare copies repeat block 0
+      const int sgb = bb / (2 * NB * S), h = (bb / (NB * S)) & 1, f = bb / S % NB, s = bb % S;
+      const int gb = gsl[sgb] + (c * 2 + h) * d.ncomp + nd0 + f;
+      copy_block(dig + (size_t)((gb * S + s) * ntile + rt) * 1024, fsrc, bufa + 1024u * b);
     }
-    int *ex = reinterpret_cast<int *>(buf + LY::NBB * 64);
+    const unsigned exa = bufa + 16u * LY::EX;  // exponents, 4 bytes each
     // column exponents [slot][sym/asym][component][16 radii] (four groups per
-    // instruction) and row exponents [P, dP, P/sin][parity][64 rows]
-    const int ntask = NB + (VEC ? 6 : 2);
-    for (int task = wave; task < ntask; task += kGemmThreads / 64) {
+    // task) and row exponents [P, dP, P/sin][parity][64 rows]
+#pragma unroll
+    for (int j = 0; j < LY::KE; j++) {
+      const int task = wave + 8 * j;
+      const int *base = tb.eP;  // spare copies
+      unsigned off = 0;
       if (task < NB) {
         const int g = task * 4 + (lane >> 4);
         const int sgb = g / (2 * NB), h = (g / NB) & 1, f = g % NB;
-        const int sl = sgb == 0 ? L + M : L - M;
-        const long long gb = (((long long)sl * nchf + c) * 2 + h) * d.ncomp + nd0 + f;
-        copy_dword(dexp + gb * d.nr_p + rt * 16 + (lane & 15), ex + task * 64);
-      } else {
+        const int gb = gsl[sgb] + (c * 2 + h) * d.ncomp + nd0 + f;
+        base = dexp;
+        off = 4u * (unsigned)(gb * d.nr_p + rt * 16 + (lane & 15));
+      } else if (task < NB + (VEC ? 6 : 2)) {
         const int tw = task - NB, tab = tw >> 1, pr = tw & 1;
         const int ipp = pr ? ip1 : ip0;
         const int row = ib * 64 + lane;
-        const int *et = tab == 0 ? tb.eP : tab == 1 ? tb.edP : tb.ePa;
-        const int *src = row < ipp ? et + tb.eoff[2 * M + pr] + (long long)c * ipp + row : et;
-        copy_dword(src, ex + 64 * NB + tw * 64);
+        base = tab == 0 ? tb.eP : tab == 1 ? tb.edP : tb.ePa;
+        off = row < ipp ? 4u * (unsigned)((pr ? erow1 : erow0) + c * ipp + row) : 0u;
       }
+      copy_dword(base, off, exa + 256u * task);
     }
   };
 
@@ -737,23 +846,31 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
       for (int r = 0; r < 4; r++) C[a][o][r] = 0.0;
 
   const int hx = par, hy = 1 - par;  // P: symmetric for even l-M; dP the other one
-  const long long abase = tb.off[2 * M + par] + ((long long)i0 + (lane & 15)) * 64 + 16 * (lane >> 4);
+  // 64-byte rows; idle waves load row 0 of the table, so that every wave
+  // issues the same vector-memory operations on every path and the counted
+  // waits hold on all paths
+  const int arow = active ? (int)(tb.off[2 * M + par] >> 6) + i0 : 0;
+  const int ipa = active ? ip : 0;
   auto load_a = [&](const signed char *tab, int c, i4 (&a)[S]) {
 #pragma unroll
-    for (int s = 0; s < S; s++)
-      a[s] = *reinterpret_cast<const i4 *>(tab + abase + ((long long)c * S + s) * ip * 64);
+    for (int s = 0; s < S; s++) a[s] = load_slice(tab + (size_t)(arow + (c * S + s) * ipa) * 64, fsrc);
   };
+  constexpr int KCOPY = LY::KB + LY::KE;  // copies per wave per step
   i4 aP[S], adP[S], aPa[S];
-  issue(0);
+  load_a(tb.P, 0, aP);
+  issue(0, 0);
   for (int c = 0; c < nchf; c++) {
     wait_copies();
     __syncthreads();
-    if (NBUF == 2 && c + 1 < nchf) issue(c + 1);
-    if (active) {
-      const i4 *buf = smem + (c % NBUF) * LY::I4;
-      const int *ex = reinterpret_cast<const int *>(buf + LY::NBB * 64);
-      load_a(tb.P, c, aP);
-      if (VEC) load_a(tb.dP, c, adP);
+    const i4 *buf = smem + (c % NBUF) * LY::I4;
+    const int *ex = reinterpret_cast<const int *>(buf + LY::EX);
+    // this step's dP slices (register loads) before the next step's copies;
+    // P was prefetched during the previous step.  The last step loads and
+    // copies itself again (into registers and the idle buffer).
+    if (VEC) load_a(tb.dP, c, adP);
+    const int cn = c + 1 < nchf ? c + 1 : c;
+    if (NBUF == 2) issue(cn, (c + 1) % NBUF);
+    {
       const int sgc0 = nsig == 1 ? 0 : 1;  // conjugate of slot 0 (slot 1's is 0)
       const int rb = 64 * NB + par * 64 + (wave & 3) * 16 + 4 * (lane >> 4);
       const int lk = lane & 15;
@@ -784,23 +901,38 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
   }
 #define AOF(q, s) AA_[s]
 #define BOF(q, t) BOFS(qq, (q), t)
+      landed(aP);  // loaded during the previous step, covered by the wait above
 #define AA_ aP
-      KSHT_FWD_QUANTITY(0, 0)
+      if (active) {
+        KSHT_FWD_QUANTITY(0, 0)
+        if (!VEC) {
+          if (NQ > 1) KSHT_FWD_QUANTITY(1 % NQ, 0)
+          if (NQ > 2) KSHT_FWD_QUANTITY(2 % NQ, 0)
+        }
+      }
 #undef AA_
+      if (!VEC) load_a(tb.P, cn, aP);  // prefetch for the next step
       if (VEC) {
+        // one table ahead: P/sin loads during the dP quantities, the next
+        // step's P during the P/sin quantities; waits leave the later loads
+        // (and the next step's copies) in flight
         load_a(tb.Pa, c, aPa);
+        wait_vm<(NBUF == 2 ? S + KCOPY : S)>();
+        landed(adP);
 #define AA_ adP
-        KSHT_FWD_QUANTITY(1 % NQ, 1)
-        KSHT_FWD_QUANTITY(3 % NQ, 1)
+        if (active) {
+          KSHT_FWD_QUANTITY(1 % NQ, 1)
+          KSHT_FWD_QUANTITY(3 % NQ, 1)
+        }
 #undef AA_
+        load_a(tb.P, cn, aP);
+        wait_vm<S>();
+        landed(aPa);
 #define AA_ aPa
-        KSHT_FWD_QUANTITY(2 % NQ, 2)
-        KSHT_FWD_QUANTITY(4 % NQ, 2)
-#undef AA_
-      } else {
-#define AA_ aP
-        if (NQ > 1) KSHT_FWD_QUANTITY(1 % NQ, 0)
-        if (NQ > 2) KSHT_FWD_QUANTITY(2 % NQ, 0)
+        if (active) {
+          KSHT_FWD_QUANTITY(2 % NQ, 2)
+          KSHT_FWD_QUANTITY(4 % NQ, 2)
+        }
 #undef AA_
       }
 #undef KSHT_FWD_QUANTITY
@@ -813,8 +945,15 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
     }
     if (NBUF == 1 && c + 1 < nchf) {
       __syncthreads();
-      issue(c + 1);
+      issue(c + 1, 0);
     }
+  }
+  // the last step's repeated loads: keep their registers until they land
+  wait_vm<0>();
+  landed(aP);
+  if (VEC) {
+    landed(adP);
+    landed(aPa);
   }
   if (!active) return;
   const int k = kcol;

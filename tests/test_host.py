@@ -157,3 +157,63 @@ def test_fortran_binding_links_and_validates(tmp_path):
     assert r.returncode == 0, r.stderr
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert "rc=1" in out.stdout, out.stdout + out.stderr
+
+
+# --- the split GEMMs' hand-counted waits (checker of the generated code) ---
+_K = "_ZN4ksht9k_leg_toyEv:\n"
+
+
+def _waits(body):
+    from kemorin_mhd_amd.isa_waits import check
+    return check(_K + body + "\ts_endpgm\n", "k_leg_")["_ZN4ksht9k_leg_toyEv"]
+
+
+def test_wait_check_accepts_counted_waits():
+    body = ("\tglobal_load_dwordx4 v[2:5], v1, s[0:1]\n"
+            "\tglobal_load_lds_dwordx4 v1, s[2:3]\n"
+            "\ts_waitcnt vmcnt(1)\n"
+            "\tv_mfma_i32_16x16x64_i8 v[6:9], v[2:5], v[10:13], 0\n")
+    assert _waits(body) == []
+
+
+def test_wait_check_flags_use_and_copy_before_the_wait():
+    use = ("\tglobal_load_dwordx4 v[2:5], v1, s[0:1]\n"
+           "\tglobal_load_lds_dwordx4 v1, s[2:3]\n"
+           "\ts_waitcnt vmcnt(2)\n"
+           "\tv_mov_b64_e32 v[20:21], v[2:3]\n")
+    assert len(_waits(use)) == 1
+    reuse = ("\tglobal_load_dwordx4 v[2:5], v1, s[0:1]\n"
+             "\tv_mov_b32_e32 v4, 0\n"
+             "\ts_waitcnt vmcnt(0)\n")
+    assert len(_waits(reuse)) == 1
+
+
+def test_wait_check_follows_every_path():
+    # the copy after the branch is skipped on one path, so vmcnt(1) does
+    # not retire the register load there
+    body = ("\tglobal_load_dwordx4 v[2:5], v1, s[0:1]\n"
+            "\ts_cbranch_scc1 .LBB0_2\n"
+            "\tglobal_load_lds_dwordx4 v1, s[2:3]\n"
+            ".LBB0_2:\n"
+            "\ts_waitcnt vmcnt(1)\n"
+            "\tv_add_u32_e32 v30, v2, v31\n")
+    assert len(_waits(body)) == 1
+    assert _waits(body.replace("vmcnt(1)", "vmcnt(0)")) == []
+
+
+def test_wait_check_flags_scratch():
+    assert any("scratch" in p for p in _waits("\tscratch_store_dword off, v4, off\n"))
+
+
+def test_wait_check_flags_missing_wait_states():
+    m0 = "\ts_mov_b32 m0, s4\n\tglobal_load_lds_dwordx4 v1, s[2:3]\n\ts_waitcnt vmcnt(0)\n"
+    assert any("M0" in p for p in _waits(m0))
+    assert _waits(m0.replace("m0, s4\n", "m0, s4\n\ts_nop 0\n")) == []
+    sg = "\tv_readfirstlane_b32 s0, v7\n\tglobal_load_lds_dword v1, s[0:1]\n\ts_waitcnt vmcnt(0)\n"
+    assert any("SGPR" in p for p in _waits(sg))
+    assert any("SGPR" in p for p in _waits(sg.replace("v7\n", "v7\n\ts_nop 3\n")))
+    assert _waits(sg.replace("v7\n", "v7\n\ts_nop 4\n")) == []
+    # across a branch target: the write sits in a predecessor block
+    br = ("\tv_readfirstlane_b32 s1, v7\n\ts_cbranch_scc1 .LBB0_1\n\ts_nop 7\n"
+          ".LBB0_1:\n\tglobal_load_dwordx4 v[2:5], v1, s[0:1]\n\ts_waitcnt vmcnt(0)\n")
+    assert any("SGPR" in p for p in _waits(br))
