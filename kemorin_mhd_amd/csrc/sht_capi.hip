@@ -46,8 +46,8 @@ static GridAddr x_grid_addr(const sht_plan *p) {
 }
 static FourierAddr x_fourier_addr(const sht_plan *p) {
   FourierAddr xa;
-  xa.q_off = p->t.q_off_X;
-  xa.q_mul = 1;
+  xa.fftpack = 0;
+  xa.q_stride = (long long)p->nr * p->nth;
   xa.s_nd = (long long)p->nslot * p->nr * p->nth;
   xa.s_line = 1;
   return xa;
@@ -63,8 +63,8 @@ static GridAddr rtm_grid_addr(const sht_plan *p, int ncomp) {
 }
 static FourierAddr rtp_fourier_addr(const sht_plan *p, int ncomp) {
   FourierAddr xa;
-  xa.q_off = p->t.q_off_rtp;
-  xa.q_mul = ncomp;
+  xa.fftpack = 1;
+  xa.q_stride = (long long)ncomp * p->nr * p->nth;
   xa.s_nd = 1;
   xa.s_line = ncomp;
   return xa;

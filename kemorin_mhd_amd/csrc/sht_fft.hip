@@ -183,13 +183,13 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_bwd(
     const long long base = (long long)nd * xa.s_nd;
     double aa = 0, ba = 0, ab = 0, bb = 0;
     if (m == 0) {
-      const long long o = xa.q_off[0] * xa.q_mul + base;
+      const long long o = q_offset(xa, 0) + base;
       if (la < nline) aa = X[o + la * xa.s_line];
       if (lb < nline) ab = X[o + lb * xa.s_line];
       buf[f * n] = make_double2(aa, ab);
     } else {
-      const long long oc = xa.q_off[2 * m - 1] * xa.q_mul + base;
-      const long long os = xa.q_off[2 * m] * xa.q_mul + base;
+      const long long oc = q_offset(xa, 2 * m - 1) + base;
+      const long long os = q_offset(xa, 2 * m) + base;
       if (la < nline) {
         aa = X[oc + la * xa.s_line];
         ba = X[os + la * xa.s_line];
@@ -261,12 +261,12 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_fwd(
     const double xbr = 0.5 * (zm.y + zn.y), xbi = -0.5 * (zm.x - zn.x);
     const long long base = (long long)nd * xa.s_nd;
     if (m == 0) {
-      const long long o = xa.q_off[0] * xa.q_mul + base;
+      const long long o = q_offset(xa, 0) + base;
       if (la < nline) X[o + la * xa.s_line] = sn * xar;
       if (lb < nline) X[o + lb * xa.s_line] = sn * xbr;
     } else {
-      const long long oc = xa.q_off[2 * m - 1] * xa.q_mul + base;
-      const long long os = xa.q_off[2 * m] * xa.q_mul + base;
+      const long long oc = q_offset(xa, 2 * m - 1) + base;
+      const long long os = q_offset(xa, 2 * m) + base;
       if (la < nline) {
         X[oc + la * xa.s_line] = tsn * xar;
         X[os + la * xa.s_line] = -tsn * xai;
@@ -384,15 +384,17 @@ __global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long
     const int q0 = f0 == 0 ? 0 : 2 * f0 - 1, nrow = 2 * (f1 - 1) - q0 + 1;
     // batches of 8 independent loads per thread, then the LDS stores
     for (int base = 0; base < nrow * 16; base += 8 * NP) {
+      // unconditional loads (clamped to a valid element, zeroed after) so
+      // the eight stay in flight together
       double val[8];
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int idx = base + tid + u * NP;
-        const int row = idx >> 4, l16 = idx & 15;
-        const long long line = line0 + l16;
-        val[u] = (idx < nrow * 16 && line < nline)
-                     ? X[xa.q_off[q0 + row] * xa.q_mul + xbase + line * xa.s_line]
-                     : 0.0;
+        const long long line = line0 + (idx & 15);
+        const bool ok = idx < nrow * 16 && line < nline;
+        const int row = ok ? idx >> 4 : 0;
+        val[u] = X[q_offset(xa, q0 + row) + xbase + (ok ? line : line0) * xa.s_line];
+        if (!ok) val[u] = 0.0;
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
@@ -479,18 +481,24 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
   double2 *ln = lds + cl * (NP + NP / 8);
   for (int j1 = 0; j1 < R0; j1++) {
     {
-      double2 z[8];  // eight independent loads per thread, then the LDS stores
+      // eight independent loads per thread, then the LDS stores; the loads
+      // are unconditional (lines clamped into range, zeroed after) so they
+      // stay in flight together
+      double2 z[8];
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int idx = tid + u * NP, j2 = idx >> 3, c = idx & 7;
         const long long la = line0 + 2 * c;
-        const double *src = v + (long long)nd * nnod + (long long)(R0 * j2 + j1) * nline + la;
-        z[u] = make_double2(0.0, 0.0);
-        if (la + 1 < nline) {
-          z[u] = vec2 ? *reinterpret_cast<const double2 *>(src) : make_double2(src[0], src[1]);
-        } else if (la < nline) {
-          z[u].x = src[0];
+        const double *row = v + (long long)nd * nnod + (long long)(R0 * j2 + j1) * nline;
+        if (vec2) {
+          // nline even: la and nline - 2 are even, so the pair is aligned
+          z[u] = *reinterpret_cast<const double2 *>(row + (la + 1 < nline ? la : nline - 2));
+        } else {
+          z[u] = make_double2(row[la < nline ? la : nline - 1],
+                              row[la + 1 < nline ? la + 1 : nline - 1]);
         }
+        if (la >= nline) z[u].x = 0.0;
+        if (la + 1 >= nline) z[u].y = 0.0;
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
@@ -504,12 +512,26 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
     for (int q = 0; q < 8; q++) w[q] = lds[(j + T * q) * 9 + cl];
     __syncthreads();
     f3_passes<NP, 1>(w, ln, j, twn, N, R0, -1.0);
+    // alias twiddles, in batches of kTwBatch orders (register pressure):
+    // loaded for every k (indices are in range for all k < NP), used for
+    // the orders that exist
+    constexpr int kTwBatch = 1;
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int k = j + T * q;
-      const double2 y = w[f3_pos<NP>(q)];
-      if (k <= L) zlo[q] = cfma(y, twn[alias_lo(k, j1)], zlo[q]);
-      if (k >= NP - L) zhi[q] = cfma(y, twn[alias_hi(k, j1, N, NP)], zhi[q]);
+    for (int h = 0; h < 8; h += kTwBatch) {
+      double2 tlo[kTwBatch], thi[kTwBatch];
+#pragma unroll
+      for (int q = 0; q < kTwBatch; q++) {
+        const int k = j + T * (h + q);
+        tlo[q] = twn[alias_lo(k, j1)];
+        thi[q] = twn[alias_hi(k, j1, N, NP)];
+      }
+#pragma unroll
+      for (int q = 0; q < kTwBatch; q++) {
+        const int k = j + T * (h + q);
+        const double2 y = w[f3_pos<NP>(h + q)];
+        if (k <= L) zlo[h + q] = cfma(y, tlo[q], zlo[h + q]);
+        if (k >= NP - L) zhi[h + q] = cfma(y, thi[q], zhi[h + q]);
+      }
     }
   }
   // Z_{N-m} for m = j + T q sits at index NP - m of the lane (T - j) % T of
@@ -556,7 +578,7 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
       const int row = idx >> 4, l16 = idx & 15;
       const long long line = line0 + l16;
       if (line < nline)
-        X[xa.q_off[q0 + row] * xa.q_mul + xbase + line * xa.s_line] = stage[row * kStageStride + l16];
+        X[q_offset(xa, q0 + row) + xbase + line * xa.s_line] = stage[row * kStageStride + l16];
     }
   }
 }

@@ -56,12 +56,18 @@ struct GridAddr {
 struct SpecAddr {
   long long s_nd, s_j, s_k;
 };
-// Addressing of an FFT-side element: q_off[q]*q_mul + nd*s_nd + line*s_line,
-// q = 0 (a0), 2m-1 (cos m), 2m (sin m), line = k + nr*t.
+// Addressing of an FFT-side element: row(q)*q_stride + nd*s_nd + line*s_line,
+// q = 0 (a0), 2m-1 (cos m), 2m (sin m), line = k + nr*t.  Rows are in Fourier
+// order (row = q: the internal layout) or in FFTPACK slot order (a0 -> 0,
+// cos m -> 2m, sin m -> 2m+1: row = q + 1 for q >= 1, the rtp layout), so the
+// row offset is arithmetic and FFT kernels need no table lookup per element.
 struct FourierAddr {
-  const long long *q_off;
-  long long q_mul, s_nd, s_line;
+  int fftpack;
+  long long q_stride, s_nd, s_line;
 };
+__host__ __device__ inline long long q_offset(const FourierAddr &xa, int q) {
+  return (long long)((xa.fftpack && q) ? q + 1 : q) * xa.q_stride;
+}
 // Grid field (v_rtp): line + nline*i + nnod*nd.
 
 struct Tables {
@@ -78,7 +84,6 @@ struct Tables {
   double2 *tw = nullptr;                        // FFT twiddles
   double2 *twn = nullptr;                       // exp(-2 pi i k / Nphi), k < Nphi
   long long *slot_off_X = nullptr, *slot_off_rtm = nullptr;  // 2L+1 each
-  long long *q_off_X = nullptr, *q_off_rtp = nullptr;        // 2L+1 each
 };
 
 }  // namespace ksht

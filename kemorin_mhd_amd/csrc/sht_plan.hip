@@ -226,7 +226,7 @@ static void free_plan(sht_plan *p) {
   void *ptrs[] = {p->t.offb, p->t.offf, p->t.ipad, p->t.jst, p->t.Pb, p->t.dPb,
                   p->t.Pf, p->t.dPf, p->t.radius, p->t.a_r, p->t.weight, p->t.asin_t,
                   p->t.cos_t, p->t.sin_t, p->t.tw, p->t.twn, p->t.slot_off_X, p->t.slot_off_rtm,
-                  p->t.q_off_X, p->t.q_off_rtp, p->X};
+                  p->X};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   split_free(p);
@@ -310,19 +310,14 @@ extern "C" int sht_plan_create(const sht_desc *d, sht_plan **out) {
   for (int k = 0; k < nr; k++) a_r[k] = 1.0 / p->radius[k];
   p->lstack.assign(p->nslot + 1, 0);
   for (int s = 0; s < p->nslot; s++) p->lstack[s + 1] = p->lstack[s] + (L - abs(s - L) + 1);
-  // slot / Fourier offset tables
+  // slot offset tables: slot s (m = s - L) sits at Fourier row q of the
+  // internal layout (q = 0 for m = 0, 2m - 1 for cos m, 2|m| for sin m)
   const long long nline = (long long)nr * nth;
-  std::vector<long long> so_X(p->nslot), so_rtm(p->nslot), qo_X(p->nslot), qo_rtp(p->nslot);
+  std::vector<long long> so_X(p->nslot), so_rtm(p->nslot);
   for (int s = 0; s < p->nslot; s++) {
     const int m = s - L;
     const int q = m == 0 ? 0 : (m > 0 ? 2 * m - 1 : 2 * (-m));
     so_X[s] = (long long)q * nline;
-    qo_X[s] = (long long)s * nline;  // indexed by q
-  }
-  for (int q = 0; q < p->nslot; q++) {
-    // FFTPACK rtp slot of Fourier index q: a0 -> 0, cos m -> 2m, sin m -> 2m+1
-    const int pslot = q == 0 ? 0 : ((q & 1) ? q + 1 : q + 1);
-    qo_rtp[q] = (long long)pslot * nline;  // scaled by ncomp at call time
   }
   for (int s = 0; s < p->nslot; s++) so_rtm[s] = (long long)s * nth * nr;  // x ncomp
   CHECK(upload(&p->t.jst, p->lstack.data(), p->nslot, &p->device_bytes));
@@ -334,8 +329,6 @@ extern "C" int sht_plan_create(const sht_desc *d, sht_plan **out) {
   CHECK(upload(&p->t.sin_t, sin_t.data(), nth, &p->device_bytes));
   CHECK(upload(&p->t.slot_off_X, so_X.data(), p->nslot, &p->device_bytes));
   CHECK(upload(&p->t.slot_off_rtm, so_rtm.data(), p->nslot, &p->device_bytes));
-  CHECK(upload(&p->t.q_off_X, qo_X.data(), p->nslot, &p->device_bytes));
-  CHECK(upload(&p->t.q_off_rtp, qo_rtp.data(), p->nslot, &p->device_bytes));
   CHECK(generate_p_tables(p));
   CHECK(fft_plan_init(p));
   p->x_elems = (long long)p->max_ncomp * p->nslot * nline;
