@@ -394,12 +394,12 @@ __global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long
         const bool ok = idx < nrow * 16 && line < nline;
         const int row = ok ? idx >> 4 : 0;
         val[u] = X[q_offset(xa, q0 + row) + xbase + (ok ? line : line0) * xa.s_line];
-        if (!ok) val[u] = 0.0;
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int idx = base + tid + u * NP;
-        if (idx < nrow * 16) stage[(idx >> 4) * kStageStride + (idx & 15)] = val[u];
+        if (idx < nrow * 16)
+          stage[(idx >> 4) * kStageStride + (idx & 15)] = line0 + (idx & 15) < nline ? val[u] : 0.0;
       }
     }
     __syncthreads();
@@ -497,13 +497,13 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
           z[u] = make_double2(row[la < nline ? la : nline - 1],
                               row[la + 1 < nline ? la + 1 : nline - 1]);
         }
-        if (la >= nline) z[u].x = 0.0;
-        if (la + 1 >= nline) z[u].y = 0.0;
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int idx = tid + u * NP;
-        lds[(idx >> 3) * 9 + (idx & 7)] = z[u];
+        const long long la = line0 + 2 * (idx & 7);
+        lds[(idx >> 3) * 9 + (idx & 7)] =
+            make_double2(la < nline ? z[u].x : 0.0, la + 1 < nline ? z[u].y : 0.0);
       }
     }
     __syncthreads();

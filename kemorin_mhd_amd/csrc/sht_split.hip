@@ -273,19 +273,22 @@ __global__ void __launch_bounds__(kSliceThreads) k_bslice(
     const int d0 = 2 * i0, nd_run = min(2 * ni, n - d0) * ncomp;
     __syncthreads();
     for (int base = 0; base < nd_run; base += 8 * kSliceThreads) {
-      double val[8];  // independent loads first, then the LDS stores
+      // independent loads first, then the LDS stores; the loads are
+      // unconditional (clamped to element 0 of the run, zeroed after) so the
+      // eight stay in flight together.  Radii k >= nr read radius nr - 1.
+      double val[8];
+      const double *srck = k < nr ? src : src - (long long)(k - nr + 1) * sa.s_k;
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int idx = base + tid + u * kSliceThreads;
-        const int dd = idx / ncomp, cq = idx - dd * ncomp;
-        val[u] = (idx < nd_run && k < nr)
-                     ? src[(long long)(d0 + dd) * sa.s_j + (long long)cq * sa.s_nd]
-                     : 0.0;
+        const int ii = idx < nd_run ? idx : 0;
+        const int dd = ii / ncomp, cq = ii - dd * ncomp;
+        val[u] = srck[(long long)(d0 + dd) * sa.s_j + (long long)cq * sa.s_nd];
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int idx = base + tid + u * kSliceThreads;
-        if (idx < nd_run) run[idx] = val[u];
+        if (idx < nd_run) run[idx] = k < nr ? val[u] : 0.0;
       }
     }
     __syncthreads();
@@ -321,7 +324,14 @@ __global__ void __launch_bounds__(kSliceThreads) k_fslice(
     GridAddr ga, const double *__restrict__ weight, signed char *__restrict__ dig,
     int *__restrict__ dexp) {
   __shared__ double sC[kFsGroup][2][64][17];  // [component][sym/asym][row][radius]
-  const int kt = blockIdx.x, sl = blockIdx.y;
+  // 1-D grid over (slot, radius tile), radius tile fastest.  Work-groups are
+  // dealt round-robin to the 8 XCDs; neighbouring radius tiles share the
+  // 128-byte lines their 16-radius rows straddle (Nr need not be a multiple
+  // of 16), so consecutive logical tiles are kept on one XCD and its L2.
+  const int ntile_g = nr_p >> 4;
+  const unsigned nwg = gridDim.x, per = nwg / 8;
+  const unsigned lb = blockIdx.x < per * 8 ? (blockIdx.x % 8) * per + blockIdx.x / 8 : blockIdx.x;
+  const int kt = (int)(lb % ntile_g), sl = (int)(lb / ntile_g);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ntile = nr_p >> 4;
   const int t_eq = (nth & 1) ? nth / 2 : -1;
@@ -339,22 +349,25 @@ __global__ void __launch_bounds__(kSliceThreads) k_fslice(
         const int kk = k_fast ? (idx & 15) : (idx >> 6);
         const int tt = k_fast ? (idx >> 4) : (idx & 63);
         const int t = c * 64 + tt, k = kt * 16 + kk;
-        vn[u] = vs[u] = 0.0;
-        if (f < ng && k < nr && t < nth_h) {
-          const long long b = base + (long long)(cg + f) * ga.s_nd + (long long)k * ga.s_k;
-          vn[u] = in[b + (long long)t * ga.s_t];
-          if (t != t_eq) vs[u] = in[b + (long long)(nth - 1 - t) * ga.s_t];
-        }
+        // unconditional loads (clamped into range, zeroed after) so that all
+        // of them stay in flight together
+        const bool ok = f < ng && k < nr && t < nth_h;
+        const int fc = ok ? f : 0, kc = ok ? k : 0, tc = ok ? t : 0;
+        const long long b = base + (long long)(cg + fc) * ga.s_nd + (long long)kc * ga.s_k;
+        vn[u] = in[b + (long long)tc * ga.s_t];
+        vs[u] = in[b + (long long)(nth - 1 - tc) * ga.s_t];
       }
 #pragma unroll
       for (int u = 0; u < kFsGroup * 4; u++) {
         const int f = u / 4, idx = tid + kSliceThreads * (u % 4);
         const int kk = k_fast ? (idx & 15) : (idx >> 6);
         const int tt = k_fast ? (idx >> 4) : (idx & 63);
-        const int t = c * 64 + tt;
-        const double w = t < nth_h ? weight[t] : 0.0;
-        sC[f][0][tt][kk] = (vn[u] + vs[u]) * w;
-        sC[f][1][tt][kk] = t != t_eq ? (vn[u] - vs[u]) * w : 0.0;
+        const int t = c * 64 + tt, k = kt * 16 + kk;
+        const bool ok = f < ng && k < nr && t < nth_h;
+        const double w = weight[t < nth_h ? t : 0];
+        const double xn = ok ? vn[u] : 0.0, xs = (ok && t != t_eq) ? vs[u] : 0.0;
+        sC[f][0][tt][kk] = (xn + xs) * w;
+        sC[f][1][tt][kk] = (xn - xs) * w;
       }
       __syncthreads();
       // wave task = (component, sym/asym, radius)
@@ -452,7 +465,19 @@ __device__ __forceinline__ double pair_scale(int g0, bool two) {
   {                                                                                    \
     _Pragma("unroll") for (int j_ = 0; j_ < 2; j_++)                                   \
       _Pragma("unroll") for (int r_ = 0; r_ < 4; r_++) T[j_][r_] = 0.0;                \
-    _Pragma("unroll") for (int g0 = 0; g0 < S; g0 += 2) {                              \
+    /* all data slices of the quantities first (one LDS latency, not one */            \
+    /* per product group; occupancy is bounded by LDS, not registers), */              \
+    /* unless the kernel has no registers to spare (BATCH_B) */                        \
+    i4 bv_[2][S];                                                                      \
+    if constexpr (BATCH_B) {                                                           \
+      _Pragma("unroll") for (int j_ = 0; j_ < NQB; j_++)                               \
+        _Pragma("unroll") for (int t = 0; t < S; t++) bv_[j_][t] = BOF((j_ ? (QB) : (QA)), t); \
+      __builtin_amdgcn_sched_group_barrier(0x100, NQB * S, 0);                         \
+    }                                                                                  \
+    /* groups from the smallest scale up (the conversions of one group */             \
+    /* overlap the products of the next) */                                            \
+    _Pragma("unroll") for (int gg = 0; gg < (S + 1) / 2; gg++) {                       \
+      const int g0 = 2 * ((S + 1) / 2 - 1 - gg);                                       \
       const bool two = g0 + 1 < S;                                                     \
       i4 a0[2], a1[2];                                                                 \
       _Pragma("unroll") for (int j_ = 0; j_ < 2; j_++) {                               \
@@ -464,7 +489,7 @@ __device__ __forceinline__ double pair_scale(int g0, bool two) {
         const int s0 = g0 - t, s1 = g0 + 1 - t;                                        \
         _Pragma("unroll") for (int j_ = 0; j_ < NQB; j_++) {                           \
           const int q = j_ ? (QB) : (QA);                                              \
-          const i4 b = BOF(q, t);                                                      \
+          const i4 b = BATCH_B ? bv_[j_][t] : BOF(q, t);                               \
           if (s0 >= 0) a0[j_] = mfma8(AOF(q, s0), b, a0[j_]);                          \
           if (two && s1 >= 0) a1[j_] = mfma8(AOF(q, s1), b, a1[j_]);                   \
         }                                                                              \
@@ -475,6 +500,11 @@ __device__ __forceinline__ double pair_scale(int g0, bool two) {
           const int v = two ? a0[j_][r_] * 256 + a1[j_][r_] : a0[j_][r_];              \
           T[j_][r_] = fma((double)v, sc, T[j_][r_]);                                   \
         }                                                                              \
+    }                                                                                  \
+    /* issue pattern: each product followed by two conversion steps */                 \
+    _Pragma("unroll") for (int i_ = 0; i_ < NQB * S * (S + 1) / 2; i_++) {             \
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                               \
+      __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);                               \
     }                                                                                  \
   }
 
@@ -509,6 +539,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
     GridAddr ga, const double *__restrict__ a_r, const double *__restrict__ asin_t) {
   extern __shared__ i4 smem[];
   using LY = BwdLds<S, NB>;
+  constexpr bool BATCH_B = true;  // data slices of a quantity read together
   constexpr int NQ = VEC ? 5 : NB;
   constexpr int NO = VEC ? 3 : NB;
   const int tid = threadIdx.x, lane = tid & 63;
@@ -748,8 +779,8 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
 //   pol = r^2 g7 [P vr], dpol = r g7 ([dP vt] - m [P/sin vp']),
 //   tor = r g7 (-[dP vp] - m [P/sin vt']); scalars g6 [P s].
 // LDS per step: the data slices of both slots and the exponents; the wave's
-// own P, dP, P/sin slices go to registers, P prefetched one step ahead and
-// dP, P/sin loaded while the preceding quantities reduce.
+// own P, dP, P/sin slices go to registers, P and dP prefetched one step
+// ahead and P/sin loaded at the top of its step.
 template <int S, int NB>
 struct FwdLds {
   static constexpr int NBB = 4 * NB * S;  // [slot][sym/asym][component][digit]
@@ -772,6 +803,9 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
   extern __shared__ i4 smem[];
   using LY = FwdLds<S, NB>;
   constexpr int NBUF = LY::NBUF;
+  // data slices of a quantity read together (not for 7-slice vectors: no
+  // registers to spare)
+  constexpr bool BATCH_B = !(VEC && S >= 7);
   constexpr int NQ = VEC ? 5 : NB;
   constexpr int NO = VEC ? 3 : NB;
   const int tid = threadIdx.x, lane = tid & 63;
@@ -857,17 +891,20 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
   };
   constexpr int KCOPY = LY::KB + LY::KE;  // copies per wave per step
   i4 aP[S], adP[S], aPa[S];
+  // P and dP slices are prefetched one step ahead; P/sin is loaded at the top
+  // of its step and first used after three quantities
   load_a(tb.P, 0, aP);
+  if (VEC) load_a(tb.dP, 0, adP);
   issue(0, 0);
   for (int c = 0; c < nchf; c++) {
     wait_copies();
     __syncthreads();
     const i4 *buf = smem + (c % NBUF) * LY::I4;
     const int *ex = reinterpret_cast<const int *>(buf + LY::EX);
-    // this step's dP slices (register loads) before the next step's copies;
-    // P was prefetched during the previous step.  The last step loads and
-    // copies itself again (into registers and the idle buffer).
-    if (VEC) load_a(tb.dP, c, adP);
+    // this step's P/sin slices (register loads) before the next step's
+    // copies.  The last step loads and copies itself again (into registers
+    // and the idle buffer).
+    if (VEC) load_a(tb.Pa, c, aPa);
     const int cn = c + 1 < nchf ? c + 1 : c;
     if (NBUF == 2) issue(cn, (c + 1) % NBUF);
     {
@@ -911,22 +948,18 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
         }
       }
 #undef AA_
-      if (!VEC) load_a(tb.P, cn, aP);  // prefetch for the next step
+      load_a(tb.P, cn, aP);  // prefetch for the next step
       if (VEC) {
-        // one table ahead: P/sin loads during the dP quantities, the next
-        // step's P during the P/sin quantities; waits leave the later loads
-        // (and the next step's copies) in flight
-        load_a(tb.Pa, c, aPa);
-        wait_vm<(NBUF == 2 ? S + KCOPY : S)>();
-        landed(adP);
+        landed(adP);  // loaded during the previous step
 #define AA_ adP
         if (active) {
           KSHT_FWD_QUANTITY(1 % NQ, 1)
           KSHT_FWD_QUANTITY(3 % NQ, 1)
         }
 #undef AA_
-        load_a(tb.P, cn, aP);
-        wait_vm<S>();
+        load_a(tb.dP, cn, adP);  // prefetch for the next step
+        // P/sin landed; the next step's copies, P and dP stay in flight
+        wait_vm<(NBUF == 2 ? KCOPY : 0) + 2 * S>();
         landed(aPa);
 #define AA_ aPa
         if (active) {
@@ -1201,7 +1234,7 @@ static int fwd_split_S(sht_plan *p, int nv, int ns, const double *in, GridAddr g
                        SpecAddr sa) {
   SplitState *st = (SplitState *)p->split;
   const int ncomp = 3 * nv + ns;
-  hipLaunchKernelGGL(k_fslice<S>, dim3(p->nr_p / 16, p->nslot), dim3(kSliceThreads), 0,
+  hipLaunchKernelGGL(k_fslice<S>, dim3((p->nr_p / 16) * p->nslot), dim3(kSliceThreads), 0,
                      p->stream, p->nr, p->nr_p, p->nth, p->nth_h, st->nchf, ncomp, in, ga,
                      p->t.weight, st->dig, st->dexp);
   KSHT_HIP(hipGetLastError());
