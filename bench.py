@@ -132,7 +132,7 @@ def run_engine(cfg, precision, args, torch, sp_b, grid_f, keep=False):
     torch.cuda.empty_cache()
     ms = (t1 - t0) * 1e3 / args.steps
     return dict(ms_per_step=ms, round_trips_per_s=1e3 / ms, timers=timers, plan_bytes=dev_bytes,
-                outputs=out)
+                outputs=out, slices=(args.slices or 6) if precision == "split" else 0)
 
 
 FIELDS_BWD = (("vector1", 0, 3), ("vector2", 3, 6), ("scalar1", 6, 7), ("scalar2", 7, 8))
@@ -179,40 +179,80 @@ def split_accuracy(cfg, args, torch, results):
     return acc
 
 
-def kernel_roofline(cfg_d, res, precision, steps):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sht_roundtrip_L511_Nr257_hbm_traffic.txt")
+
+
+def hbm_traffic(engine, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE / WRITE_SIZE, scaled by a device-copy calibration), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("#"):
+                    continue
+                parts = [x.strip() for x in line.split("|")]
+                if len(parts) == 6 and parts[0] == engine and parts[1] == kernel:
+                    return float(parts[5])
+    except OSError:
+        return None
+    return None
+
+
+def kernel_roofline(cfg_d, res, precision, slices):
+    """Roofline of the round trip's dominant kernel.  Legendre kernels: the
+    vector-group kernel alone (stage timers 5/6, HIP events on the plan's
+    stream), algorithmic work = FP64 flops of its contractions (10 F1 backward:
+    2 vectors x 5, 15 F1 forward: 3 vectors x 5); in the split engine each FP64
+    multiply-add is S(S+1)/2 INT8 slice products, so its INT8 work is that
+    many times the FP64 flops, against the INT8 MFMA peak.  FFT stages: the
+    bytes a launch must move (X in, grid out, or the reverse)."""
     F1 = flops_per_contraction(cfg_d)
     nline = cfg_d["nr"] * cfg_d["nth"]
     nslot = 2 * cfg_d["L"] + 1
+    tri = slices * (slices + 1) // 2
+    npow = min(cfg_d["nphi"] & -cfg_d["nphi"], 512)  # Nphi = R0 * NP, NP a power of two
+    split = precision == "split"
+    kname = {
+        "legendre_backward_vector_kernel": (f"ksht::k_leg_bwd_split<{slices}, 3, true>" if split
+                                            else "ksht::k_leg_bwd<true>"),
+        "legendre_forward_vector_kernel": (f"ksht::k_leg_fwd_split<{slices}, 3, true>" if split
+                                           else "ksht::k_leg_fwd<true>"),
+        "fft_backward": f"ksht::k_fft3_bwd<{npow}>",
+        "fft_forward": f"ksht::k_fft3_fwd<{npow}>",
+    }
     stages = {
-        "legendre_backward": ("mfma", 12 * F1),
-        "legendre_forward": ("mfma", 15 * F1),
+        "legendre_backward_vector_kernel": ("mfma", 10 * F1),
+        "legendre_forward_vector_kernel": ("mfma", 15 * F1),
         "fft_backward": ("hbm", 8.0 * 8 * nline * (nslot + cfg_d["nphi"])),
         "fft_forward": ("hbm", 8.0 * 9 * nline * (nslot + cfg_d["nphi"])),
     }
     t = res["timers"]
-    dom = max(stages, key=lambda s: t[s][0])
+    avg = {s: t[s][0] / max(t[s][1], 1) for s in stages}
+    dom = max(stages, key=lambda s: avg[s])
     bound, work = stages[dom]
-    tot_ms, calls = t[dom]
-    avg_s = tot_ms / max(calls, 1) / 1e3
+    avg_s = avg[dom] / 1e3
     if bound == "mfma":
-        achieved = work / avg_s / 1e12
-        if precision == "split":
-            peak, unit = INT8_PEAK_TOPS, "TOP/s (FP64-equivalent work / INT8 peak)"
+        if split:
+            work = work * tri
+            achieved, peak, unit = work / avg_s / 1e12, INT8_PEAK_TOPS, "TOP/s (INT8 slice products)"
         else:
-            peak, unit = FP64_PEAK_TFLOPS, "TFLOP/s"
+            achieved, peak, unit = work / avg_s / 1e12, FP64_PEAK_TFLOPS, "TFLOP/s"
     else:
-        achieved = work / avg_s / 1e9
-        peak, unit = HBM_PEAK_GBS, "GB/s"
+        achieved, peak, unit = work / avg_s / 1e9, HBM_PEAK_GBS, "GB/s"
+    traffic = hbm_traffic(precision, kname[dom]) if cfg_d == CONFIGS["cfg2"] else None
     per_stage = {s: {"ms_per_launch": t[s][0] / max(t[s][1], 1), "launches": t[s][1]}
-                 for s in stages}
+                 for s in t}
     for s, (b, w) in stages.items():
-        a = t[s][0] / max(t[s][1], 1) / 1e3
+        a = avg[s] / 1e3
         if a > 0:
+            per_stage[s]["kernel"] = kname[s]
             per_stage[s]["achieved"] = (w / a / 1e12) if b == "mfma" else (w / a / 1e9)
             per_stage[s]["unit"] = "TFLOP/s (FP64-equivalent)" if b == "mfma" else "GB/s"
-    return {"kernel": dom, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-            "frac": achieved / peak, "traffic": None,
-            "work_per_launch": work}, per_stage
+    return {"kernel": kname[dom], "bound": bound, "achieved": achieved, "peak": peak,
+            "unit": unit, "frac": achieved / peak, "traffic": traffic,
+            "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
+                            "copy-calibrated)",
+            "work_per_launch": work, "avg_ms_per_launch": avg[dom]}, per_stage
 
 
 def cpu_baseline(cfg_d, nr_sample):
@@ -291,7 +331,7 @@ def main():
         print(f"# split accuracy: {json.dumps(accuracy)}", file=sys.stderr, flush=True)
     head = "split" if "split" in results else "fp64"
     res = results[head]
-    roof, per_stage = kernel_roofline(cfg_d, res, head, args.steps)
+    roof, per_stage = kernel_roofline(cfg_d, res, head, res["slices"])
     F_rt = 27 * flops_per_contraction(cfg_d)
     out = {
         "metric": METRIC, "value": res["round_trips_per_s"], "unit": "round_trips/s",

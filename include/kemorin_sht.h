@@ -76,9 +76,11 @@ long long sht_plan_device_bytes(const sht_plan *plan);
  * 22 Legendre backward, 23 Legendre forward, 24 FFT, 18-21 exchanges).
  * When enabled, every kernel launch of the plan is bracketed by HIP events
  * on the plan's stream.  Stage ids: 0 Legendre backward, 1 FFT backward,
- * 2 FFT forward, 3 Legendre forward, 4 exchanges.
+ * 2 FFT forward, 3 Legendre forward, 4 exchanges; 5 and 6 time the
+ * vector-group Legendre kernel alone (backward, forward; inside 0 and 3,
+ * without the operand slicing of the split engine).
  * sht_plan_timer_read synchronises, returns per-stage total milliseconds and
- * launch counts (n entries each, n <= 5) and resets the counters. */
+ * launch counts (n entries each, n <= 7) and resets the counters. */
 int sht_plan_timer_enable(sht_plan *plan, int on);
 int sht_plan_timer_read(sht_plan *plan, double *ms, long long *calls, int n);
 

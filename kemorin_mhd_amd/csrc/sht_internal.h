@@ -18,7 +18,9 @@ typedef int i4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTile = 16;   // MFMA tile edge (rows, radii, degrees)
 constexpr int kMaxPass = 24;
-constexpr int kNumStages = 5;  // Legendre bwd, FFT bwd, FFT fwd, Legendre fwd, exchanges
+// Legendre bwd, FFT bwd, FFT fwd, Legendre fwd, exchanges, and the vector-group
+// Legendre kernels alone (bwd, fwd): the dominant kernels of a round trip
+constexpr int kNumStages = 7;
 
 void set_error(const char *fmt, ...);
 

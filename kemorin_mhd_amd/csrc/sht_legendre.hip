@@ -452,11 +452,12 @@ int launch_leg_bwd(sht_plan *p, int nv, int ns, const double *sp, SpecAddr sa, d
     d.ncg = ncg;
     const long long nblocks = (long long)(p->L + 1) * d.n_rt * d.ncg * d.nblk;
     const int cg0 = vec ? 0 : nv;
-    if (vec)
+    if (vec) {
+      StageTimer tk(p, 5);
       hipLaunchKernelGGL(k_leg_bwd<true>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d, cg0,
                          sp, sa, out, ga, p->t.jst, p->t.offb, p->t.ipad, p->t.Pb, p->t.dPb,
                          p->t.a_r, p->t.asin_t);
-    else
+    } else
       hipLaunchKernelGGL(k_leg_bwd<false>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d,
                          cg0, sp, sa, out, ga, p->t.jst, p->t.offb, p->t.ipad, p->t.Pb, p->t.dPb,
                          p->t.a_r, p->t.asin_t);
@@ -476,11 +477,12 @@ int launch_leg_fwd(sht_plan *p, int nv, int ns, const double *in, GridAddr ga, d
     d.ncg = ncg;
     const long long nblocks = (long long)(p->L + 1) * d.n_rt * d.ncg * d.nblk;
     const int cg0 = vec ? 0 : nv;
-    if (vec)
+    if (vec) {
+      StageTimer tk(p, 6);
       hipLaunchKernelGGL(k_leg_fwd<true>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d, cg0,
                          in, ga, sp, sa, p->t.jst, p->t.offf, p->t.ipad, p->t.Pf, p->t.dPf,
                          p->t.radius, p->t.weight, p->t.asin_t);
-    else
+    } else
       hipLaunchKernelGGL(k_leg_fwd<false>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d,
                          cg0, in, ga, sp, sa, p->t.jst, p->t.offf, p->t.ipad, p->t.Pf, p->t.dPf,
                          p->t.radius, p->t.weight, p->t.asin_t);

@@ -1215,9 +1215,10 @@ static int bwd_split_S(sht_plan *p, int nv, int ns, const double *spec, SpecAddr
   hipLaunchKernelGGL((k_leg_bwd_split<S, NB, VEC>), grid, blk, bwd_lds<S>(NB), p->stream, d, \
                      g[i].nd_first, tb, st->cb, st->dig, st->dexp, out, ga, p->t.a_r,        \
                      p->t.asin_t)
-    if (g[i].vec)
+    if (g[i].vec) {
+      StageTimer tk(p, 5);
       KSHT_BWD(3, true);
-    else if (g[i].nb == 3)
+    } else if (g[i].nb == 3)
       KSHT_BWD(3, false);
     else if (g[i].nb == 2)
       KSHT_BWD(2, false);
@@ -1251,9 +1252,10 @@ static int fwd_split_S(sht_plan *p, int nv, int ns, const double *in, GridAddr g
   hipLaunchKernelGGL((k_leg_fwd_split<S, NB, VEC>), grid, blk, fwd_lds<S>(NB), p->stream, d, \
                      g[i].nd_first, tb, st->dig, st->dexp, sp, sa, p->t.jst, p->t.ipad,      \
                      p->t.radius)
-    if (g[i].vec)
+    if (g[i].vec) {
+      StageTimer tk(p, 6);
       KSHT_FWD(3, true);
-    else if (g[i].nb == 3)
+    } else if (g[i].nb == 3)
       KSHT_FWD(3, false);
     else if (g[i].nb == 2)
       KSHT_FWD(2, false);

@@ -174,16 +174,18 @@ class SphTransPlan:
     def set_stream(self, stream_ptr: int | None):
         _check(lib().sht_plan_set_stream(self._h, stream_ptr), "sht_plan_set_stream")
 
-    STAGES = ("legendre_backward", "fft_backward", "fft_forward", "legendre_forward", "exchange")
+    STAGES = ("legendre_backward", "fft_backward", "fft_forward", "legendre_forward", "exchange",
+              "legendre_backward_vector_kernel", "legendre_forward_vector_kernel")
 
     def timer_enable(self, on: bool = True):
         _check(lib().sht_plan_timer_enable(self._h, int(on)), "sht_plan_timer_enable")
 
     def timer_read(self) -> dict:
         """{stage: (total_ms, launches)} since the last read (synchronises)."""
-        ms = (ctypes.c_double * 5)()
-        calls = (ctypes.c_longlong * 5)()
-        _check(lib().sht_plan_timer_read(self._h, ms, calls, 5), "sht_plan_timer_read")
+        n = len(self.STAGES)
+        ms = (ctypes.c_double * n)()
+        calls = (ctypes.c_longlong * n)()
+        _check(lib().sht_plan_timer_read(self._h, ms, calls, n), "sht_plan_timer_read")
         return {s: (ms[i], int(calls[i])) for i, s in enumerate(self.STAGES)}
 
     def device_bytes(self) -> int:
