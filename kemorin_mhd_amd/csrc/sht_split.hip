@@ -316,8 +316,11 @@ __global__ void __launch_bounds__(kSliceThreads) k_bslice(
 // the smaller of the radius and colatitude strides) as the weighted symmetric
 // and antisymmetric combinations (v(t) +- v(nth-1-t)) w(t), as in the FP64
 // kernel (an odd grid's equator row is its own mirror: (v w, 0)); each wave
-// then slices (component, radius, sym/asym) columns with lane = row.
-constexpr int kFsGroup = 3;
+// then slices one (component, sym/asym) block of 64 rows x 16 radii: lane
+// (radius c16 = lane % 16, rows 16 (lane / 16) .. +15) takes the column
+// exponent over its 16 rows and the 4 lanes of its radius, and stores each
+// digit's 16 bytes with one 16-byte store (one whole block per instruction).
+constexpr int kFsGroup = 2;  // 2 components x sym/asym = one block per wave
 template <int S>
 __global__ void __launch_bounds__(kSliceThreads) k_fslice(
     int nr, int nr_p, int nth, int nth_h, int nchf, int ncomp, const double *__restrict__ in,
@@ -370,15 +373,34 @@ __global__ void __launch_bounds__(kSliceThreads) k_fslice(
         sC[f][1][tt][kk] = (xn - xs) * w;
       }
       __syncthreads();
-      // wave task = (component, sym/asym, radius)
-      for (int task = wave; task < ng * 2 * 16; task += kSliceThreads / 64) {
-        const int kk = task & 15, h = (task >> 4) & 1, f = task >> 5;
-        const double x = sC[f][h][lane][kk];
-        const int e = wave_max(exp_of(x));
+      // wave task = (component, sym/asym) block
+      for (int task = wave; task < ng * 2; task += kSliceThreads / 64) {
+        const int h = task & 1, f = task >> 1;
+        const int c16 = lane & 15, r0 = 16 * (lane >> 4);
+        double x[16];
+        int e = kEmpty;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+          x[i] = sC[f][h][r0 + i][c16];
+          e = max(e, exp_of(x[i]));
+        }
+        e = max(e, __shfl_xor(e, 16));
+        e = max(e, __shfl_xor(e, 32));
+        long long F[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) F[i] = fix_of<S>(x[i], e);
         const long long g = (((long long)sl * nchf + c) * 2 + h) * ncomp + cg + f;
-        put_digits<S>(dig + (g * S * ntile + kt) * 1024, sstride, slice_off(kk, lane),
-                      fix_of<S>(x, e));
-        if (lane == 0) dexp[g * nr_p + kt * 16 + kk] = e;
+        signed char *blk = dig + (g * S * ntile + kt) * 1024 + slice_off(c16, r0);
+#pragma unroll
+        for (int sd = 0; sd < S; sd++) {
+          uint4 w4;
+          w4.x = pack4<S>(F + 0, sd);
+          w4.y = pack4<S>(F + 4, sd);
+          w4.z = pack4<S>(F + 8, sd);
+          w4.w = pack4<S>(F + 12, sd);
+          *reinterpret_cast<uint4 *>(blk + sd * sstride) = w4;
+        }
+        if (lane < 16) dexp[g * nr_p + kt * 16 + c16] = e;
       }
     }
   }
