@@ -245,66 +245,93 @@ __device__ __forceinline__ void put_digits(signed char *blk, long long sstride, 
 constexpr int kSliceThreads = 256;
 
 // ---------------------------------------------------------------------------
-// Backward data operand.  Work-group = (radius k < nr_p, slot m + L); per
-// chunk of pair indices [i0, i1) the slot's spectra at radius k for degrees
-// l - M in [2 i0, 2 i1) (one contiguous run of at most 128 ncomp doubles)
-// are staged in LDS with coalesced loads.  Wave task = (parity, component): lane
-// = pair index i of l = M + 2i + parity; the wave takes the chunk exponent
-// and writes S digit bytes per lane in fragment order plus the exponent.  The
-// first component of each vector (pol) is scaled by g3 = l(l+1) (1/2 at
-// l = 0) as in the FP64 kernels; radii nr <= k < nr_p get zero digits.
+// Backward data operand.  Work-group = (nrad <= 4 consecutive radii, slot
+// m + L; nrad = 4 unless the staging would pass 64 KiB of LDS); per
+// chunk of pair indices [i0, i1) the slot's spectra at those radii for
+// degrees l - M in [2 i0, 2 i1) (one contiguous run of at most 128 ncomp
+// doubles per radius) are staged in LDS with coalesced loads.  Wave task =
+// (parity, component): lane (radius r4 = lane / 16, pair indices
+// 4 (lane % 16) .. +3 of l = M + 2i + parity) takes the chunk exponent over
+// the 16 lanes of its radius and writes each digit's 4 bytes with one store
+// (4 rows of a block per instruction) plus the exponent.  The first component
+// of each vector (pol) is scaled by g3 = l(l+1) (1/2 at l = 0) as in the FP64
+// kernels; radii nr <= k < nr_p get zero digits.
+constexpr int kBsRadii = 4;  // lanes per wave: 4 radii x 16
 template <int S>
 __global__ void __launch_bounds__(kSliceThreads) k_bslice(
-    int L, int nr, int nr_p, int nv, int ncomp, const double *__restrict__ sp, SpecAddr sa,
+    int L, int nr, int nr_p, int nrad, int nv, int ncomp, const double *__restrict__ sp, SpecAddr sa,
     const int *__restrict__ jst, const long long *__restrict__ cb, signed char *__restrict__ dig,
     int *__restrict__ dexp) {
-  extern __shared__ double run[];  // [128 degrees][ncomp]
-  const int k = blockIdx.x, sl = blockIdx.y;
+  extern __shared__ double run[];  // [nrad][128 degrees][ncomp]
+  const int kq = blockIdx.x, sl = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int M = sl >= L ? sl - L : L - sl;
   const int n = L - M + 1, np0 = (n + 1) / 2, np1 = n / 2;
   const int nch0 = bchunk_count(M, np0), nch1 = bchunk_count(M, np1);
   const int ntile = nr_p >> 4;
-  const int off = slice_off(k & 15, lane);
   const long long sstride = (long long)ntile * 1024;
-  const double *src = sp + (long long)jst[sl] * sa.s_j + (long long)k * sa.s_k;
+  const int k0 = kq * nrad;
+  const double *src = sp + (long long)jst[sl] * sa.s_j;
+  const int r4 = lane >> 4, q16 = lane & 15;
+  const int k = k0 + r4;  // this lane's radius in the slicing phase
   for (int c = 0; c < nch0; c++) {
     const int i0 = bchunk_start(M, c), ni = bchunk_end(M, c) - i0;
     const int d0 = 2 * i0, nd_run = min(2 * ni, n - d0) * ncomp;
+    const int nall = nrad * nd_run;
     __syncthreads();
-    for (int base = 0; base < nd_run; base += 8 * kSliceThreads) {
+    for (int base = 0; base < nall; base += 8 * kSliceThreads) {
       // independent loads first, then the LDS stores; the loads are
-      // unconditional (clamped to element 0 of the run, zeroed after) so the
-      // eight stay in flight together.  Radii k >= nr read radius nr - 1.
+      // unconditional (clamped to a valid element, masked at the store) so
+      // the eight stay in flight together.  Radii >= nr read radius nr - 1.
       double val[8];
-      const double *srck = k < nr ? src : src - (long long)(k - nr + 1) * sa.s_k;
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int idx = base + tid + u * kSliceThreads;
-        const int ii = idx < nd_run ? idx : 0;
-        const int dd = ii / ncomp, cq = ii - dd * ncomp;
-        val[u] = srck[(long long)(d0 + dd) * sa.s_j + (long long)cq * sa.s_nd];
+        const int ii = idx < nall ? idx : 0;
+        const int rr = ii / nd_run, e = ii - rr * nd_run;
+        const int dd = e / ncomp, cq = e - dd * ncomp;
+        const int kr = min(k0 + rr, nr - 1);
+        val[u] = src[(long long)kr * sa.s_k + (long long)(d0 + dd) * sa.s_j + (long long)cq * sa.s_nd];
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int idx = base + tid + u * kSliceThreads;
-        if (idx < nd_run) run[idx] = k < nr ? val[u] : 0.0;
+        if (idx < nall) {
+          const int rr = idx / nd_run, e = idx - rr * nd_run;
+          run[rr * 128 * ncomp + e] = k0 + rr < nr ? val[u] : 0.0;
+        }
       }
     }
     __syncthreads();
     const int ntask = (c < nch1 ? 2 : 1) * ncomp;
     for (int task = wave; task < ntask; task += kSliceThreads / 64) {
       const int par = task >= ncomp ? 1 : 0, cq = task - par * ncomp;
-      const int i = i0 + lane;
-      double x = 0.0;
-      if (lane < ni && i < (par ? np1 : np0)) {
-        x = run[(2 * lane + par) * ncomp + cq];
-        if (cq < 3 * nv && cq % 3 == 0) x *= g3s_of(M + 2 * i + par);
+      const int npar = par ? np1 : np0;
+      double x[4];
+      int e = kEmpty;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int il = 4 * q16 + j, i = i0 + il;
+        x[j] = 0.0;
+        if (r4 < nrad && il < ni && i < npar) {
+          x[j] = run[r4 * 128 * ncomp + (2 * il + par) * ncomp + cq];
+          if (cq < 3 * nv && cq % 3 == 0) x[j] *= g3s_of(M + 2 * i + par);
+        }
+        e = max(e, exp_of(x[j]));
       }
-      const int e = wave_max(exp_of(x));
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) e = max(e, __shfl_xor(e, o));
+      long long F[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) F[j] = fix_of<S>(x[j], e);
       const long long gb = (cb[2 * sl + par] + c) * ncomp + cq;
-      put_digits<S>(dig + (gb * S * ntile + (k >> 4)) * 1024, sstride, off, fix_of<S>(x, e));
-      if (lane == 0) dexp[gb * nr_p + k] = e;
+      signed char *blk = dig + (gb * S * ntile + (k >> 4)) * 1024 + slice_off(k & 15, 4 * q16);
+      if (r4 < nrad) {
+#pragma unroll
+        for (int sd = 0; sd < S; sd++)
+          *reinterpret_cast<unsigned *>(blk + sd * sstride) = pack4<S>(F, sd);
+        if (q16 == 0) dexp[gb * nr_p + k] = e;
+      }
     }
   }
 }
@@ -1220,9 +1247,12 @@ static int bwd_split_S(sht_plan *p, int nv, int ns, const double *spec, SpecAddr
                        GridAddr ga) {
   SplitState *st = (SplitState *)p->split;
   const int ncomp = 3 * nv + ns;
-  hipLaunchKernelGGL(k_bslice<S>, dim3(p->nr_p, p->nslot), dim3(kSliceThreads),
-                     sizeof(double) * 128 * ncomp, p->stream,
-                     p->L, p->nr, p->nr_p, nv, ncomp, spec, sa, p->t.jst, st->cb, st->dig,
+  // radii per work-group: 4, unless the staging [nrad][128][ncomp] doubles
+  // would pass 64 KiB of LDS (nr_p is a multiple of 16)
+  const int nrad = ncomp <= 16 ? 4 : ncomp <= 32 ? 2 : 1;
+  hipLaunchKernelGGL(k_bslice<S>, dim3(p->nr_p / nrad, p->nslot), dim3(kSliceThreads),
+                     sizeof(double) * nrad * 128 * ncomp, p->stream,
+                     p->L, p->nr, p->nr_p, nrad, nv, ncomp, spec, sa, p->t.jst, st->cb, st->dig,
                      st->dexp);
   KSHT_HIP(hipGetLastError());
   SplitTabs tb{st->b_off, st->b_eoff, st->bP, st->bdP, nullptr, st->beP, st->bedP, nullptr};
