@@ -87,6 +87,16 @@ def _check(rc: int, where: str):
         raise ShtError(rc, where)
 
 
+def _torch_stream(*bufs):
+    """Handle of torch's current stream when any buffer is a torch device
+    tensor, else None."""
+    for t in bufs:
+        if hasattr(t, "is_cuda") and t.is_cuda:
+            import torch
+            return torch.cuda.current_stream(t.device).cuda_stream
+    return None
+
+
 def _ptr(t) -> int:
     """Device (or host numpy) address of a contiguous float64 buffer."""
     if isinstance(t, np.ndarray):
@@ -155,6 +165,7 @@ class SphTransPlan:
         self._h = h
         self.max_ncomp = max_ncomp
         self.precision = precision
+        self._user_stream = False
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
@@ -172,7 +183,19 @@ class SphTransPlan:
         return self._h
 
     def set_stream(self, stream_ptr: int | None):
+        """Run the plan on this HIP stream from now on (None: back to the
+        plan's own stream and to following torch's current stream)."""
         _check(lib().sht_plan_set_stream(self._h, stream_ptr), "sht_plan_set_stream")
+        self._user_stream = stream_ptr is not None
+
+    def _bind(self, *bufs):
+        """Order a call on torch device tensors like a torch operation: on
+        torch's current stream, unless set_stream chose one."""
+        if self._user_stream:
+            return
+        s = _torch_stream(*bufs)
+        if s is not None:
+            _check(lib().sht_plan_set_stream(self._h, s), "sht_plan_set_stream")
 
     STAGES = ("legendre_backward", "fft_backward", "fft_forward", "legendre_forward", "exchange",
               "legendre_backward_vector_kernel", "legendre_forward_vector_kernel")
@@ -207,11 +230,13 @@ class SphTransPlan:
     # whole transforms -------------------------------------------------------
     def sph_backward_transforms(self, nvector: int, nscalar: int, WS, v_rtp):
         """spectra (rj send buffer) -> grid fields, device buffers."""
+        self._bind(WS, v_rtp)
         _check(lib().sht_backward(self._h, nvector, nscalar, _ptr(WS), _ptr(v_rtp)),
                "sht_backward")
 
     def sph_forward_transforms(self, nvector: int, nscalar: int, v_rtp, WR):
         """grid fields -> spectra (rj receive buffer), device buffers."""
+        self._bind(v_rtp, WR)
         _check(lib().sht_forward(self._h, nvector, nscalar, _ptr(v_rtp), _ptr(WR)),
                "sht_forward")
 
@@ -233,29 +258,36 @@ class SphTransPlan:
 
     # stages -------------------------------------------------------------------
     def legendre_backward(self, nvector, nscalar, WR_rlm, WS_rtm):
+        self._bind(WR_rlm, WS_rtm)
         _check(lib().sht_legendre_backward(self._h, nvector, nscalar, _ptr(WR_rlm), _ptr(WS_rtm)),
                "sht_legendre_backward")
 
     def legendre_forward(self, nvector, nscalar, WR_rtm, WS_rlm):
+        self._bind(WR_rtm, WS_rlm)
         _check(lib().sht_legendre_forward(self._h, nvector, nscalar, _ptr(WR_rtm), _ptr(WS_rlm)),
                "sht_legendre_forward")
 
     def fft_backward(self, ncomp, WR_rtp, v_rtp):
+        self._bind(WR_rtp, v_rtp)
         _check(lib().sht_fft_backward(self._h, ncomp, _ptr(WR_rtp), _ptr(v_rtp)), "sht_fft_backward")
 
     def fft_forward(self, ncomp, v_rtp, WS_rtp):
+        self._bind(v_rtp, WS_rtp)
         _check(lib().sht_fft_forward(self._h, ncomp, _ptr(v_rtp), _ptr(WS_rtp)), "sht_fft_forward")
 
     def exchange_rtm_to_rtp(self, ncomp, rtm, rtp):
+        self._bind(rtm, rtp)
         _check(lib().sht_exchange_rtm_to_rtp(self._h, ncomp, _ptr(rtm), _ptr(rtp)),
                "sht_exchange_rtm_to_rtp")
 
     def exchange_rtp_to_rtm(self, ncomp, rtp, rtm):
+        self._bind(rtp, rtm)
         _check(lib().sht_exchange_rtp_to_rtm(self._h, ncomp, _ptr(rtp), _ptr(rtm)),
                "sht_exchange_rtp_to_rtm")
 
     def exchange_gather_rev(self, ncomp, n_dst, n_src, irev, src, dst):
         assert irev.is_contiguous()
+        self._bind(src, dst)
         _check(lib().sht_exchange_gather_rev(self._h, ncomp, n_dst, n_src, irev.data_ptr(),
                                              _ptr(src), _ptr(dst)), "sht_exchange_gather_rev")
 
