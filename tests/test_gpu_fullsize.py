@@ -202,3 +202,43 @@ def test_cfg2_fp64_linearity(torch_cuda):
         plan.close()
         del a, b
         t.cuda.empty_cache()
+
+
+def test_truncation_L1023(torch_cuda):
+    """Config 4's truncation L=1023 on the smallest grid it allows (Ntheta =
+    L+1 = 1024, Nphi = 2L+2 = 2048: past the split-form FFT, so the
+    mixed-radix FFT runs) and 2 radii, to keep the host's O(Ntheta^3) Gauss
+    points short: FP64 engine against the oracle (1e-12 per component),
+    split engine against the FP64 engine (1e-11), whole backward (2 vector +
+    2 scalar) and forward (3 vector)."""
+    t = torch_cuda
+    cfg = ShellConfig(1023, 2, 1024, 2048)
+    _say("[gpu] L=1023 Ntheta=1024 Nphi=2048, 2 radii")
+    orc = O.OraclePlan(cfg.L, cfg.nr, cfg.nth, cfg.nphi, cfg.radius(), full_tables=False)
+    sp = spectra(cfg, 8, 401)
+    spf = spectra(cfg, 9, 403)
+    ref_v = orc.backward(2, 2, sp)
+    grid = orc.backward(3, 0, spf)
+    ref_s = orc.forward(3, 0, grid)
+    _say("    oracle done")
+    outs = {}
+    for prec in (SHT_PREC_FP64, SHT_PREC_SPLIT_INT8):
+        plan = SphTransPlan(cfg, max_ncomp=9, precision=prec)
+        _say(f"    plan {prec} ready")
+        try:
+            v = t.empty(cfg.n_rtp(8), dtype=t.float64, device="cuda")
+            s = t.empty(cfg.n_rj(9), dtype=t.float64, device="cuda")
+            plan.sph_backward_transforms(2, 2, t.from_numpy(sp).cuda(), v)
+            plan.sph_forward_transforms(3, 0, t.from_numpy(grid).cuda(), s)
+            t.cuda.synchronize()
+            outs[prec] = (v.cpu().numpy(), s.cpu().numpy())
+        finally:
+            plan.close()
+    eb = rel_comp_np(outs[SHT_PREC_FP64][0], ref_v, 8, comp_fastest=False)
+    ef = rel_comp_np(outs[SHT_PREC_FP64][1], ref_s, 9, comp_fastest=True)
+    sb = rel_comp_np(outs[SHT_PREC_SPLIT_INT8][0], outs[SHT_PREC_FP64][0], 8, comp_fastest=False)
+    sf = rel_comp_np(outs[SHT_PREC_SPLIT_INT8][1], outs[SHT_PREC_FP64][1], 9, comp_fastest=True)
+    _say(f"    FP64 vs oracle: backward {eb:.3e} forward {ef:.3e}; "
+         f"split vs FP64: backward {sb:.3e} forward {sf:.3e}")
+    assert eb <= 1e-12 and ef <= 1e-12
+    assert sb <= 1e-11 and sf <= 1e-11
