@@ -22,7 +22,8 @@
  * All pointers passed to the transform entry points are DEVICE pointers
  * (hipMalloc'd or torch CUDA tensors) unless the name ends in _host.
  * Calls on one plan are stream-ordered on the plan's stream and not
- * re-entrant; different plans may run concurrently.
+ * re-entrant; different plans may run concurrently.  A plan's own stream is
+ * a blocking stream, ordered with the legacy default stream.
  */
 #ifndef KEMORIN_SHT_H
 #define KEMORIN_SHT_H

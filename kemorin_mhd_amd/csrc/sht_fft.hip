@@ -377,8 +377,53 @@ __global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long
   double2 zlo[8], zhi[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) zlo[q] = zhi[q] = make_double2(0.0, 0.0);
-  // coefficients in chunks of 256 orders (rows 2 f0 - 1 .. 2 f0 + 510)
   const long long xbase = (long long)nd * xa.s_nd;
+  if (xa.s_line == 1 && (nline & 1) == 0 && (xa.s_nd & 1) == 0 && (xa.q_stride & 1) == 0 &&
+      ((unsigned long long)X & 15) == 0) {
+    // Lines contiguous, rows 16-byte aligned (even strides, aligned base):
+    // the whole coefficient block
+    // (rows q = 0 .. 2L, 16 lines of 8 bytes each) goes straight to LDS with
+    // global_load_lds_dwordx4, one wait for all of it.  Row r sits at 128 r
+    // bytes; its 16-byte slot u holds lines 2 (u ^ sw(r)) and +1, sw(r) =
+    // (r >> 1) & 7, so the reads below (rows 2f - 1, 2f of consecutive
+    // orders f) fall in distinct slots.  Rows are padded to 2 NP; lines past
+    // the end read a valid pair (their results are never stored).
+    double2 *st2 = lds;
+    const int nrow = 2 * L + 1;
+    for (int i0 = tid & ~63; i0 < 16 * NP; i0 += NP) {
+      const int i = i0 + (tid & 63);
+      const int r = i >> 3, u = i & 7;
+      const int c = u ^ ((r >> 1) & 7);
+      long long line = line0 + 2 * c;
+      if (line + 1 >= nline) line = line0;
+      const double *src = X + q_offset(xa, r < nrow ? r : 0) + xbase + line;
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                       (__attribute__((address_space(3))) void *)(st2 + i0), 16,
+                                       0, 0);
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    auto row2 = [&](int r) { return st2[r * 8 + (cl ^ ((r >> 1) & 7))]; };
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int f = j + T * q;  // order of Z_f; Z_{f+N-NP} comes from order NP - f
+      if (f <= L) {
+        if (f == 0) {
+          zlo[q] = row2(0);
+        } else {
+          const double2 a = row2(2 * f - 1), b = row2(2 * f);
+          zlo[q] = make_double2(0.5 * (a.x + b.y), 0.5 * (a.y - b.x));
+        }
+      }
+      const int g = NP - f;
+      if (g <= L && g >= 1) {
+        const double2 a = row2(2 * g - 1), b = row2(2 * g);
+        zhi[q] = make_double2(0.5 * (a.x - b.y), 0.5 * (a.y + b.x));
+      }
+    }
+    __syncthreads();
+  } else
+  // coefficients in chunks of 256 orders (rows 2 f0 - 1 .. 2 f0 + 510)
   for (int f0 = 0; f0 <= L; f0 += 256) {
     const int f1 = min(f0 + 256, L + 1);
     const int q0 = f0 == 0 ? 0 : 2 * f0 - 1, nrow = 2 * (f1 - 1) - q0 + 1;
@@ -652,6 +697,8 @@ int fft_plan_init(sht_plan *p) {
     const int rows = 2 * std::min(256, p->L + 1);
     fp.lds3 = (int)std::max({sizeof(double) * rows * kStageStride,
                              sizeof(double2) * 8 * (np + np / 8), sizeof(double2) * np * 9});
+    // backward: the whole coefficient block of 16 lines, 2 np rows of 128 bytes
+    fp.lds3b = std::max(fp.lds3, (int)(sizeof(double2) * 8 * 2 * np));
     std::vector<double2> twn(n);
     for (int k = 0; k < n; k++) {
       const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
@@ -663,7 +710,7 @@ int fft_plan_init(sht_plan *p) {
 #define KSHT_FFT3_ATTR(NP)                                                                   \
   case NP:                                                                                   \
     KSHT_HIP(hipFuncSetAttribute((const void *)k_fft3_bwd<NP>,                               \
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, fp.lds3));      \
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, fp.lds3b));     \
     KSHT_HIP(hipFuncSetAttribute((const void *)k_fft3_fwd<NP>,                               \
                                  hipFuncAttributeMaxDynamicSharedMemorySize, fp.lds3));      \
     break;
@@ -680,15 +727,15 @@ int fft_plan_init(sht_plan *p) {
   return SHT_OK;
 }
 
-#define KSHT_FFT3_LAUNCH(K, ...)                                                             \
+#define KSHT_FFT3_LAUNCH(K, LDS, ...)                                                        \
   do {                                                                                     \
     const dim3 g3((unsigned)((nline + 15) / 16), (unsigned)ncomp);                         \
     switch (p->fft.np) {                                                                   \
-      case 32: hipLaunchKernelGGL((K<32>), g3, dim3(32), p->fft.lds3, p->stream, __VA_ARGS__); break; \
-      case 64: hipLaunchKernelGGL((K<64>), g3, dim3(64), p->fft.lds3, p->stream, __VA_ARGS__); break; \
-      case 128: hipLaunchKernelGGL((K<128>), g3, dim3(128), p->fft.lds3, p->stream, __VA_ARGS__); break; \
-      case 256: hipLaunchKernelGGL((K<256>), g3, dim3(256), p->fft.lds3, p->stream, __VA_ARGS__); break; \
-      default: hipLaunchKernelGGL((K<512>), g3, dim3(512), p->fft.lds3, p->stream, __VA_ARGS__); break; \
+      case 32: hipLaunchKernelGGL((K<32>), g3, dim3(32), (LDS), p->stream, __VA_ARGS__); break; \
+      case 64: hipLaunchKernelGGL((K<64>), g3, dim3(64), (LDS), p->stream, __VA_ARGS__); break; \
+      case 128: hipLaunchKernelGGL((K<128>), g3, dim3(128), (LDS), p->stream, __VA_ARGS__); break; \
+      case 256: hipLaunchKernelGGL((K<256>), g3, dim3(256), (LDS), p->stream, __VA_ARGS__); break; \
+      default: hipLaunchKernelGGL((K<512>), g3, dim3(512), (LDS), p->stream, __VA_ARGS__); break; \
     }                                                                                      \
   } while (0)
 
@@ -712,7 +759,7 @@ int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, doub
     }                                                                                      \
   } while (0)
   if (p->fft.fast)
-    KSHT_FFT3_LAUNCH(k_fft3_bwd, p->nphi, p->fft.r0, p->L, nline, X, xa, v, p->t.twn);
+    KSHT_FFT3_LAUNCH(k_fft3_bwd, p->fft.lds3b, p->nphi, p->fft.r0, p->L, nline, X, xa, v, p->t.twn);
   else
     KSHT_FFT_LAUNCH(k_fft_bwd, p->fft, p->L, nline, X, xa, v, p->t.tw);
   KSHT_HIP(hipGetLastError());
@@ -725,7 +772,7 @@ int launch_fft_fwd(sht_plan *p, int ncomp, const double *v, double *X, FourierAd
   const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);
   const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.n;
   if (p->fft.fast)
-    KSHT_FFT3_LAUNCH(k_fft3_fwd, p->nphi, p->fft.r0, p->L, nline, v, X, xa, p->t.twn);
+    KSHT_FFT3_LAUNCH(k_fft3_fwd, p->fft.lds3, p->nphi, p->fft.r0, p->L, nline, v, X, xa, p->t.twn);
   else
     KSHT_FFT_LAUNCH(k_fft_fwd, p->fft, p->L, nline, v, X, xa, p->t.tw);
   KSHT_HIP(hipGetLastError());

@@ -43,7 +43,7 @@ struct FftPlan {
   int n, npass, lines_per_wg, pad;  // pad: LDS elements per work-group
   FftPass pass[kMaxPass];
   // split form n = r0 * np (np a power of two above L): 16 lines per work-group
-  int fast, np, r0, lds3;
+  int fast, np, r0, lds3, lds3b;  // LDS bytes: forward, backward (whole-block staging)
 };
 
 // Addressing of an (order m, colatitude t, radius k, component nd) element

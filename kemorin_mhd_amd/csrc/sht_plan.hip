@@ -285,8 +285,11 @@ extern "C" int sht_plan_create(const sht_desc *d, sht_plan **out) {
       return rc;          \
     }                     \
   } while (0)
+  // the plan's own stream is a blocking stream: it is ordered with the legacy
+  // default stream, so callers that work on the default stream (torch's
+  // default stream included) need no explicit synchronisation around calls
   if (hipGetDevice(&p->device) != hipSuccess ||
-      hipStreamCreateWithFlags(&p->own_stream, hipStreamNonBlocking) != hipSuccess) {
+      hipStreamCreateWithFlags(&p->own_stream, hipStreamDefault) != hipSuccess) {
     set_error("sht_plan_create: no HIP device available");
     p->own_stream = nullptr;
     free_plan(p);

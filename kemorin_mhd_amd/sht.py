@@ -89,7 +89,7 @@ def _check(rc: int, where: str):
 
 def _torch_stream(*bufs):
     """Handle of torch's current stream when any buffer is a torch device
-    tensor, else None."""
+    tensor (0 for the default stream), else None."""
     for t in bufs:
         if hasattr(t, "is_cuda") and t.is_cuda:
             import torch
@@ -189,13 +189,15 @@ class SphTransPlan:
         self._user_stream = stream_ptr is not None
 
     def _bind(self, *bufs):
-        """Order a call on torch device tensors like a torch operation: on
-        torch's current stream, unless set_stream chose one."""
+        """Order a call on torch device tensors like a torch operation, unless
+        set_stream chose a stream: on torch's current stream, or for torch's
+        default stream (handle 0) on the plan's own stream, a blocking stream
+        that the default stream is ordered with."""
         if self._user_stream:
             return
         s = _torch_stream(*bufs)
         if s is not None:
-            _check(lib().sht_plan_set_stream(self._h, s), "sht_plan_set_stream")
+            _check(lib().sht_plan_set_stream(self._h, s or None), "sht_plan_set_stream")
 
     STAGES = ("legendre_backward", "fft_backward", "fft_forward", "legendre_forward", "exchange",
               "legendre_backward_vector_kernel", "legendre_forward_vector_kernel")
