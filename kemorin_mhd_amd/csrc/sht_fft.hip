@@ -297,9 +297,11 @@ constexpr int kStageStride = 17;  // doubles per staged row of 16 lines
 
 __device__ __forceinline__ int fpad(int i) { return i + (i >> 3); }
 
+// tw: roots of unity of order NP at stride ts (the plan's N-point table with
+// ts = R0, or an NP-entry copy in LDS with ts = 1)
 template <int NP, int Ns>
 __device__ __forceinline__ void f3_passes(double2 (&v)[8], double2 *ln, int j,
-                                          const double2 *__restrict__ twn, int N, int R0,
+                                          const double2 *__restrict__ tw, int N, int ts,
                                           double s) {
   constexpr int T = NP / 8;
   constexpr int R = (NP / Ns >= 8) ? 8 : NP / Ns;
@@ -310,8 +312,8 @@ __device__ __forceinline__ void f3_passes(double2 (&v)[8], double2 *ln, int j,
       const int k = b % Ns;
 #pragma unroll
       for (int r = 1; r < R; r++) {
-        // exp(-2 pi i k r / (Ns R)) = twn[k r (NP / (Ns R)) R0]; k r < Ns R, so < N
-        double2 w = twn[k * r * (NP / (Ns * R)) * R0];
+        // exp(-2 pi i k r / (Ns R)) = tw[k r (NP / (Ns R)) ts]; k r < Ns R
+        double2 w = tw[k * r * (NP / (Ns * R)) * ts];
         w.y *= -s;
         v[i * R + r] = cmul(v[i * R + r], w);
       }
@@ -332,7 +334,7 @@ __device__ __forceinline__ void f3_passes(double2 (&v)[8], double2 *ln, int j,
 #pragma unroll
       for (int r = 0; r < R2; r++) v[i * R2 + r] = ln[fpad(j + T * i + (NP / R2) * r)];
     __syncthreads();
-    f3_passes<NP, Ns * R>(v, ln, j, twn, N, R0, s);
+    f3_passes<NP, Ns * R>(v, ln, j, tw, N, ts, s);
   }
 }
 
@@ -357,6 +359,22 @@ __device__ __forceinline__ constexpr int f3_pos(int q) {
 __device__ __forceinline__ int alias_lo(int m, int j1) { return m * j1; }
 __device__ __forceinline__ int alias_hi(int m, int j1, int N, int NP) {
   return j1 == 0 ? 0 : m * j1 + N - NP * j1;
+}
+
+// LDS DMA of 16 bytes per lane (global_load_lds_dwordx4) from a wave-uniform
+// base plus a 32-bit lane offset, as inline assembly: the compiler would wait
+// for a builtin DMA before every later LDS access, which would serialise the
+// copy with the FFT passes it is meant to overlap.  The statement carries its
+// own wait states (the compiler's hazard recognizer does not look inside):
+// s_nop 4 covers both an SALU write of M0 and a VALU write of the base SGPRs.
+__device__ __forceinline__ void fft_dma16(const void *base, unsigned voff, unsigned lds_byte) {
+  const unsigned long long b = (unsigned long long)base;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  const void *ub = (const void *)(((unsigned long long)hi << 32) | lo);
+  asm volatile("s_nop 4\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(voff), "s"(ub),
+               "{m0}"(__builtin_amdgcn_readfirstlane(lds_byte))
+               : "memory");
 }
 
 __device__ __forceinline__ double2 cfma(double2 a, double2 w, double2 acc) {
@@ -524,6 +542,56 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
 #pragma unroll
   for (int q = 0; q < 8; q++) zlo[q] = zhi[q] = make_double2(0.0, 0.0);
   double2 *ln = lds + cl * (NP + NP / 8);
+  // DMA form (lines contiguous in pairs, aligned base, per-component byte
+  // offsets within 32 bits): the rows of the next phi decimation j1 + 1 are
+  // copied global -> LDS while the FFT of j1 runs.  LDS: [pass area: 8 (NP +
+  // NP/8)][stage: NP rows x 8 slots][NP twiddles]; row r's slot u holds
+  // lines 2 (u ^ (r & 7)) and +1 (XOR swizzle: the 64 lanes of a complex
+  // line read 8 distinct slots per 8 rows).  The pass twiddles come from the
+  // LDS copy, so the passes issue no global load whose wait would also wait
+  // for the copy in flight.
+  const bool dma = vec2 && (((unsigned long long)v & 15) == 0) && nnod * 8 < (1ll << 32);
+  if (dma) {
+    double2 *st2 = lds + 8 * (NP + NP / 8);
+    double2 *twl = st2 + 8 * NP;
+    const unsigned st2_byte =
+        (unsigned)(size_t)(const __attribute__((address_space(3))) void *)lds +
+        16u * (8 * (NP + NP / 8));
+    auto issue = [&](int jj) {
+      const double *base = v + (long long)nd * nnod + (long long)jj * nline + line0;
+#pragma unroll
+      for (int i0 = tid & ~63; i0 < 8 * NP; i0 += NP) {
+        const int i = i0 + (tid & 63), r = i >> 3, u = i & 7, c = u ^ (r & 7);
+        const long long cc = line0 + 2 * c + 1 < nline ? 2 * c : 0;  // past the end: line0
+        const unsigned off = (unsigned)(((long long)R0 * r * nline + cc) * 8);
+        fft_dma16(base, off, st2_byte + 16u * (unsigned)i0);
+      }
+    };
+    twl[tid] = twn[tid * R0];  // NP threads, NP roots of order NP
+    issue(0);
+    for (int j1 = 0; j1 < R0; j1++) {
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();
+      double2 w[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int r = j + T * q;
+        w[q] = st2[r * 8 + (cl ^ (r & 7))];
+      }
+      __syncthreads();
+      if (j1 + 1 < R0) issue(j1 + 1);
+      f3_passes<NP, 1>(w, ln, j, twl, N, 1, -1.0);
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int k = j + T * q;
+        const double2 y = w[f3_pos<NP>(q)];
+        const double2 tlo = twn[alias_lo(k, j1)], thi = twn[alias_hi(k, j1, N, NP)];
+        if (k <= L) zlo[q] = cfma(y, tlo, zlo[q]);
+        if (k >= NP - L) zhi[q] = cfma(y, thi, zhi[q]);
+      }
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  } else
   for (int j1 = 0; j1 < R0; j1++) {
     {
       // eight independent loads per thread, then the LDS stores; the loads
@@ -699,6 +767,8 @@ int fft_plan_init(sht_plan *p) {
                              sizeof(double2) * 8 * (np + np / 8), sizeof(double2) * np * 9});
     // backward: the whole coefficient block of 16 lines, 2 np rows of 128 bytes
     fp.lds3b = std::max(fp.lds3, (int)(sizeof(double2) * 8 * 2 * np));
+    // forward: pass area, a stage of np rows and np twiddles
+    fp.lds3 = std::max(fp.lds3, (int)(sizeof(double2) * (8 * (np + np / 8) + 9 * np)));
     std::vector<double2> twn(n);
     for (int k = 0; k < n; k++) {
       const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
