@@ -448,13 +448,14 @@ __global__ void __launch_bounds__(kSliceThreads) k_fslice(
 constexpr int kGemmThreads = 512;
 
 // Work-group order of the GEMMs.  The hardware deals work-groups round-robin
-// to the 8 XCDs, each with its own L2.  Logical ids are grouped in runs of G
-// (one (M, row block) with all its radius tiles and component groups, which
-// share the P slices and, across component groups or neighbouring radius
-// tiles, output lines); run r goes to XCD r % 8, so each L2 sees whole runs
-// while the 8 XCDs work through neighbouring orders at the same time (the
-// work per order falls with M, so contiguous M ranges per XCD would be
-// unbalanced).  Ids past the last full set of 8 runs keep their value.
+// to the 8 XCDs, each with its own L2, and dispatches them in order, so an
+// XCD with more work than the others holds up the whole grid.  Logical ids
+// are grouped in runs of G work-groups that share operands through L2; run r
+// goes to XCD r % 8, so each L2 sees whole runs while the 8 XCDs work
+// through neighbouring orders at the same time (the work per order falls
+// with M, so contiguous M ranges per XCD would be unbalanced; consecutive
+// runs must be consecutive orders for the same reason).  Ids past the last
+// full set of 8 runs keep their value.
 __device__ __forceinline__ long long xcd_group_order(long long b, long long n, long long G) {
   const long long sup = 8 * G, nsup = n / sup;
   if (b >= nsup * sup) return b;
@@ -608,14 +609,16 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
   constexpr int NO = VEC ? 3 : NB;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform per wave
-  // logical id = ((M nblk + tbk) ncg + cgl) n_rt + rt
-  long long bidx = xcd_group_order(blockIdx.x, gridDim.x, (long long)d.n_rt * d.ncg);
-  const int rt = (int)(bidx % d.n_rt);
-  bidx /= d.n_rt;
+  // logical id = ((M n_rt + rt) ncg + cgl) nblk + tbk: a run is a whole
+  // order, so the theta blocks that read the same data slices (and the
+  // component groups that write the same output lines) are resident together
+  long long bidx = xcd_group_order(blockIdx.x, gridDim.x, (long long)d.n_rt * d.ncg * d.nblk);
+  const int tbk = (int)(bidx % d.nblk);
+  bidx /= d.nblk;
   const int cgl = (int)(bidx % d.ncg);
   bidx /= d.ncg;
-  const int tbk = (int)(bidx % d.nblk);
-  const int M = (int)(bidx / d.nblk);
+  const int rt = (int)(bidx % d.n_rt);
+  const int M = (int)(bidx / d.n_rt);
   const int L = d.L;
   const int n = L - M + 1, np0 = (n + 1) / 2, np1 = n / 2;
   const int nch0 = bchunk_count(M, np0), nch1 = bchunk_count(M, np1);
@@ -875,14 +878,17 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
   constexpr int NO = VEC ? 3 : NB;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform per wave
-  // logical id = ((M nblk + ib) n_rt + rt) ncg + cgl
+  // logical id = ((ib (L+1) + M) n_rt + rt) ncg + cgl: a run is one (M, row
+  // block); the run index ib (L+1) + M puts consecutive orders, not row
+  // blocks, on consecutive XCDs (row blocks past the degrees of M are empty,
+  // and the work of row block ib would otherwise all land on two XCDs)
   long long bidx = xcd_group_order(blockIdx.x, gridDim.x, (long long)d.n_rt * d.ncg);
   const int cgl = (int)(bidx % d.ncg);
   bidx /= d.ncg;
   const int rt = (int)(bidx % d.n_rt);
   bidx /= d.n_rt;
-  const int ib = (int)(bidx % d.nblk);
-  const int M = (int)(bidx / d.nblk);
+  const int M = (int)(bidx % (d.L + 1));
+  const int ib = (int)(bidx / (d.L + 1));
   const int L = d.L;
   const int ip0 = ipad[2 * M], ip1 = ipad[2 * M + 1];
   if (ib * 64 >= ip0) return;  // whole work-group beyond the degrees of M
