@@ -963,20 +963,21 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
   };
   constexpr int KCOPY = LY::KB + LY::KE;  // copies per wave per step
   i4 aP[S], adP[S], aPa[S];
-  // P and dP slices are prefetched one step ahead; P/sin is loaded at the top
-  // of its step and first used after three quantities
+  // P and dP slices are prefetched one step ahead (issued after their last
+  // use), P/sin at the end of the step before (first used after three
+  // quantities, so it stays in flight across the top-of-step wait)
   load_a(tb.P, 0, aP);
   if (VEC) load_a(tb.dP, 0, adP);
   issue(0, 0);
+  if (VEC) load_a(tb.Pa, 0, aPa);
   for (int c = 0; c < nchf; c++) {
-    wait_copies();
+    // everything but the P/sin slices of this step (issued last)
+    wait_vm<VEC ? S : 0>();
     __syncthreads();
     const i4 *buf = smem + (c % NBUF) * LY::I4;
     const int *ex = reinterpret_cast<const int *>(buf + LY::EX);
-    // this step's P/sin slices (register loads) before the next step's
-    // copies.  The last step loads and copies itself again (into registers
-    // and the idle buffer).
-    if (VEC) load_a(tb.Pa, c, aPa);
+    // The last step loads and copies itself again (into registers and the
+    // idle buffer).
     const int cn = c + 1 < nchf ? c + 1 : c;
     if (NBUF == 2) issue(cn, (c + 1) % NBUF);
     {
@@ -1052,6 +1053,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
       __syncthreads();
       issue(c + 1, 0);
     }
+    if (VEC) load_a(tb.Pa, cn, aPa);  // the next step's, in flight across its barrier
   }
   // the last step's repeated loads: keep their registers until they land
   wait_vm<0>();
