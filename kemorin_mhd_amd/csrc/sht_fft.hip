@@ -297,6 +297,24 @@ constexpr int kStageStride = 17;  // doubles per staged row of 16 lines
 
 __device__ __forceinline__ int fpad(int i) { return i + (i >> 3); }
 
+// LDS layout of the split-form kernels, in double2 units: the twiddle table
+// (all N roots exp(-2 pi i k / N), copied from global memory at the start)
+// sits past every other area, so that the passes and the alias sums read no
+// global memory: a global load there would be counted with the copies or
+// stores in flight (vmcnt completes in issue order) and wait for them.
+//   backward: coefficient block (16 NP: 2 NP rows of 8 slots) or staging of
+//             256-order chunks (17 min(256, NP)), then pass / output areas
+//             (reused);
+//   forward:  pass area 8 (NP + NP/8), stage of NP rows x 8 slots.
+template <int NP>
+constexpr int fft3_tw_off_bwd() {
+  constexpr int a = 16 * NP, b = 17 * (NP < 256 ? NP : 256), c = 8 * (NP + NP / 8), d = 9 * NP;
+  constexpr int ab = a > b ? a : b, cd = c > d ? c : d;
+  return ab > cd ? ab : cd;
+}
+template <int NP>
+constexpr int fft3_tw_off_fwd() { return 8 * (NP + NP / 8) + 8 * NP; }
+
 // tw: roots of unity of order NP at stride ts (the plan's N-point table with
 // ts = R0, or an NP-entry copy in LDS with ts = 1)
 template <int NP, int Ns>
@@ -392,6 +410,8 @@ __global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long
   const int nd = blockIdx.y;
   const long long line0 = (long long)blockIdx.x * 16;
   double *stage = reinterpret_cast<double *>(lds);
+  double2 *twl = lds + fft3_tw_off_bwd<NP>();
+  for (int i = tid; i < N; i += NP) twl[i] = twn[i];  // read after the staging barrier
   double2 zlo[8], zhi[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) zlo[q] = zhi[q] = make_double2(0.0, 0.0);
@@ -495,13 +515,13 @@ __global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const int m = j + T * q;
-      double2 t1 = twn[alias_lo(m, j1)];
-      double2 t2 = twn[alias_hi(m, j1, N, NP)];
+      double2 t1 = twl[alias_lo(m, j1)];
+      double2 t2 = twl[alias_hi(m, j1, N, NP)];
       t1.y = -t1.y;  // e^{+...}
       t2.y = -t2.y;
       w[q] = cfma(zhi[q], t2, cmul(zlo[q], t1));
     }
-    f3_passes<NP, 1>(w, ln, j, twn, N, R0, 1.0);
+    f3_passes<NP, 1>(w, ln, j, twl, N, R0, 1.0);
     // rows phi = R0 (j + T q) + j1, staged as [j2][8 complex lines] (stride 9)
 #pragma unroll
     for (int q = 0; q < 8; q++) lds[(j + T * q) * 9 + cl] = w[f3_pos<NP>(q)];
@@ -542,18 +562,17 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
 #pragma unroll
   for (int q = 0; q < 8; q++) zlo[q] = zhi[q] = make_double2(0.0, 0.0);
   double2 *ln = lds + cl * (NP + NP / 8);
+  double2 *twl = lds + fft3_tw_off_fwd<NP>();
+  for (int i = tid; i < N; i += NP) twl[i] = twn[i];  // read after the first barrier
   // DMA form (lines contiguous in pairs, aligned base, per-component byte
   // offsets within 32 bits): the rows of the next phi decimation j1 + 1 are
   // copied global -> LDS while the FFT of j1 runs.  LDS: [pass area: 8 (NP +
-  // NP/8)][stage: NP rows x 8 slots][NP twiddles]; row r's slot u holds
-  // lines 2 (u ^ (r & 7)) and +1 (XOR swizzle: the 64 lanes of a complex
-  // line read 8 distinct slots per 8 rows).  The pass twiddles come from the
-  // LDS copy, so the passes issue no global load whose wait would also wait
-  // for the copy in flight.
+  // NP/8)][stage: NP rows x 8 slots][twiddles]; row r's slot u holds lines
+  // 2 (u ^ (r & 7)) and +1 (XOR swizzle: the 64 lanes of a complex line read
+  // 8 distinct slots per 8 rows).
   const bool dma = vec2 && (((unsigned long long)v & 15) == 0) && nnod * 8 < (1ll << 32);
   if (dma) {
     double2 *st2 = lds + 8 * (NP + NP / 8);
-    double2 *twl = st2 + 8 * NP;
     const unsigned st2_byte =
         (unsigned)(size_t)(const __attribute__((address_space(3))) void *)lds +
         16u * (8 * (NP + NP / 8));
@@ -567,7 +586,6 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
         fft_dma16(base, off, st2_byte + 16u * (unsigned)i0);
       }
     };
-    twl[tid] = twn[tid * R0];  // NP threads, NP roots of order NP
     issue(0);
     for (int j1 = 0; j1 < R0; j1++) {
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -580,12 +598,12 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
       }
       __syncthreads();
       if (j1 + 1 < R0) issue(j1 + 1);
-      f3_passes<NP, 1>(w, ln, j, twl, N, 1, -1.0);
+      f3_passes<NP, 1>(w, ln, j, twl, N, R0, -1.0);
 #pragma unroll
       for (int q = 0; q < 8; q++) {
         const int k = j + T * q;
         const double2 y = w[f3_pos<NP>(q)];
-        const double2 tlo = twn[alias_lo(k, j1)], thi = twn[alias_hi(k, j1, N, NP)];
+        const double2 tlo = twl[alias_lo(k, j1)], thi = twl[alias_hi(k, j1, N, NP)];
         if (k <= L) zlo[q] = cfma(y, tlo, zlo[q]);
         if (k >= NP - L) zhi[q] = cfma(y, thi, zhi[q]);
       }
@@ -624,27 +642,14 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
 #pragma unroll
     for (int q = 0; q < 8; q++) w[q] = lds[(j + T * q) * 9 + cl];
     __syncthreads();
-    f3_passes<NP, 1>(w, ln, j, twn, N, R0, -1.0);
-    // alias twiddles, in batches of kTwBatch orders (register pressure):
-    // loaded for every k (indices are in range for all k < NP), used for
-    // the orders that exist
-    constexpr int kTwBatch = 1;
+    f3_passes<NP, 1>(w, ln, j, twl, N, R0, -1.0);
 #pragma unroll
-    for (int h = 0; h < 8; h += kTwBatch) {
-      double2 tlo[kTwBatch], thi[kTwBatch];
-#pragma unroll
-      for (int q = 0; q < kTwBatch; q++) {
-        const int k = j + T * (h + q);
-        tlo[q] = twn[alias_lo(k, j1)];
-        thi[q] = twn[alias_hi(k, j1, N, NP)];
-      }
-#pragma unroll
-      for (int q = 0; q < kTwBatch; q++) {
-        const int k = j + T * (h + q);
-        const double2 y = w[f3_pos<NP>(h + q)];
-        if (k <= L) zlo[h + q] = cfma(y, tlo[q], zlo[h + q]);
-        if (k >= NP - L) zhi[h + q] = cfma(y, thi[q], zhi[h + q]);
-      }
+    for (int q = 0; q < 8; q++) {
+      const int k = j + T * q;
+      const double2 y = w[f3_pos<NP>(q)];
+      const double2 tlo = twl[alias_lo(k, j1)], thi = twl[alias_hi(k, j1, N, NP)];
+      if (k <= L) zlo[q] = cfma(y, tlo, zlo[q]);
+      if (k >= NP - L) zhi[q] = cfma(y, thi, zhi[q]);
     }
   }
   // Z_{N-m} for m = j + T q sits at index NP - m of the lane (T - j) % T of
@@ -762,13 +767,23 @@ int fft_plan_init(sht_plan *p) {
     fp.fast = 1;
     fp.np = np;
     fp.r0 = n / np;
-    const int rows = 2 * std::min(256, p->L + 1);
-    fp.lds3 = (int)std::max({sizeof(double) * rows * kStageStride,
-                             sizeof(double2) * 8 * (np + np / 8), sizeof(double2) * np * 9});
-    // backward: the whole coefficient block of 16 lines, 2 np rows of 128 bytes
-    fp.lds3b = std::max(fp.lds3, (int)(sizeof(double2) * 8 * 2 * np));
-    // forward: pass area, a stage of np rows and np twiddles
-    fp.lds3 = std::max(fp.lds3, (int)(sizeof(double2) * (8 * (np + np / 8) + 9 * np)));
+    int tw_b = 0, tw_f = 0;
+    switch (np) {
+      case 32: tw_b = fft3_tw_off_bwd<32>(); tw_f = fft3_tw_off_fwd<32>(); break;
+      case 64: tw_b = fft3_tw_off_bwd<64>(); tw_f = fft3_tw_off_fwd<64>(); break;
+      case 128: tw_b = fft3_tw_off_bwd<128>(); tw_f = fft3_tw_off_fwd<128>(); break;
+      case 256: tw_b = fft3_tw_off_bwd<256>(); tw_f = fft3_tw_off_fwd<256>(); break;
+      default: tw_b = fft3_tw_off_bwd<512>(); tw_f = fft3_tw_off_fwd<512>(); break;
+    }
+    fp.lds3b = (int)sizeof(double2) * (tw_b + n);
+    fp.lds3 = (int)sizeof(double2) * (tw_f + n);
+    int dev = 0, lds_max = 0;
+    KSHT_HIP(hipGetDevice(&dev));
+    KSHT_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (fp.lds3b > lds_max || fp.lds3 > lds_max) fp.fast = 0;  // table does not fit: generic form
+  }
+  if (fp.fast) {
+    const int np = fp.np;
     std::vector<double2> twn(n);
     for (int k = 0; k < n; k++) {
       const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
