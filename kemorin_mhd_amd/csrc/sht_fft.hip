@@ -315,8 +315,19 @@ constexpr int fft3_tw_off_bwd() {
 template <int NP>
 constexpr int fft3_tw_off_fwd() { return 8 * (NP + NP / 8) + 8 * NP; }
 
-// tw: roots of unity of order NP at stride ts (the plan's N-point table with
-// ts = R0, or an NP-entry copy in LDS with ts = 1)
+// The pass area of a complex line belongs to the wave that transforms it
+// (T = NP/8 <= 64 threads per line, lines never straddle a wave), and one
+// wave's LDS operations complete in issue order: between the passes only
+// the compiler has to keep the stores before the loads and the loads before
+// the next stores.  Callers put a work-group barrier between the passes and
+// any use of the area by other waves.
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// tw: roots of unity of order NP at stride ts (the plan's N-point table in
+// LDS with ts = R0)
 template <int NP, int Ns>
 __device__ __forceinline__ void f3_passes(double2 (&v)[8], double2 *ln, int j,
                                           const double2 *__restrict__ tw, int N, int ts,
@@ -345,13 +356,13 @@ __device__ __forceinline__ void f3_passes(double2 (&v)[8], double2 *ln, int j,
 #pragma unroll
       for (int r = 0; r < R; r++) ln[fpad((b / Ns) * Ns * R + b % Ns + Ns * r)] = v[i * R + r];
     }
-    __syncthreads();
+    wave_lds_order();
     constexpr int R2 = (NP / (Ns * R) >= 8) ? 8 : NP / (Ns * R);
 #pragma unroll
     for (int i = 0; i < 8 / R2; i++)
 #pragma unroll
       for (int r = 0; r < R2; r++) v[i * R2 + r] = ln[fpad(j + T * i + (NP / R2) * r)];
-    __syncthreads();
+    wave_lds_order();
     f3_passes<NP, Ns * R>(v, ln, j, tw, N, ts, s);
   }
 }
@@ -522,6 +533,7 @@ __global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long
       w[q] = cfma(zhi[q], t2, cmul(zlo[q], t1));
     }
     f3_passes<NP, 1>(w, ln, j, twl, N, R0, 1.0);
+    __syncthreads();  // every wave done with its pass area (under the staging)
     // rows phi = R0 (j + T q) + j1, staged as [j2][8 complex lines] (stride 9)
 #pragma unroll
     for (int q = 0; q < 8; q++) lds[(j + T * q) * 9 + cl] = w[f3_pos<NP>(q)];
@@ -611,6 +623,7 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   } else
   for (int j1 = 0; j1 < R0; j1++) {
+    __syncthreads();  // every wave done with its pass area (under the stage)
     {
       // eight independent loads per thread, then the LDS stores; the loads
       // are unconditional (lines clamped into range, zeroed after) so they
