@@ -839,7 +839,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
 
 // ---------------------------------------------------------------------------
 // Forward.  Work-group = (M, radius tile, component group, block of 64 pair
-// indices); wave w owns rows i in [64 ib + 16 (w%4), +16) of parity w/4 for
+// indices); wave w owns rows i in [64 ib + 16 (w/2), +16) of parity w%2 for
 // both slots, over the chunks of 64 hemisphere rows.  Rows of even l-M take
 // the symmetric combinations against P and P/sin, the antisymmetric ones
 // against dP; odd l-M the other way round.  Per slot (m = +-M, primes on the
@@ -893,8 +893,11 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
   const int ip0 = ipad[2 * M], ip1 = ipad[2 * M + 1];
   if (ib * 64 >= ip0) return;  // whole work-group beyond the degrees of M
   const int nsig = M == 0 ? 1 : 2;
-  const int par = wave >> 2;
-  const int i0 = ib * 64 + (wave & 3) * 16;
+  // wave w: parity w & 1, row tile w >> 1.  Waves go to the four SIMDs
+  // round-robin, so the idle tiles of a last, partly filled row block fall
+  // on different SIMDs instead of leaving two SIMDs without work
+  const int par = wave & 1, tile = wave >> 1;
+  const int i0 = ib * 64 + tile * 16;
   const int ip = par ? ip1 : ip0;
   const bool active = i0 < ip;
   const int nd0 = nd_first + NB * cgl;
@@ -982,7 +985,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
     if (NBUF == 2) issue(cn, (c + 1) % NBUF);
     {
       const int sgc0 = nsig == 1 ? 0 : 1;  // conjugate of slot 0 (slot 1's is 0)
-      const int rb = 64 * NB + par * 64 + (wave & 3) * 16 + 4 * (lane >> 4);
+      const int rb = 64 * NB + par * 64 + tile * 16 + 4 * (lane >> 4);
       const int lk = lane & 15;
       // quantity q of slot sg: data block (slot, sym/asym, component)
 #define QSLOT(q, sg) ((VEC && ((q) == 2 || (q) == 4)) ? ((sg) == 0 ? sgc0 : 0) : (sg))
