@@ -281,9 +281,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_fwd(
 
 // ---------------------------------------------------------------------------
 // Split form for Nphi = R0 * NP with NP a power of two, L < NP (the usual
-// Nphi = 3 * 2^k grids).  A work-group owns 16 consecutive lines (8 complex
-// lines of two packed real lines) so every global access moves whole
-// 128-byte rows of the line-contiguous layouts.  With j = R0 j2 + j1:
+// Nphi = 3 * 2^k grids).  With j = R0 j2 + j1:
 //   backward  x_j = Re/Im of sum_m Z_m e^{+2 pi i m j/N}
 //                 = sum_{m'<NP} D^{(j1)}_m' e^{+2 pi i m' j2/NP},
 //             D^{(j1)}_m' = e^{2 pi i m' j1/N} Z_m' + e^{2 pi i (m'+N-NP) j1/N} Z_{m'+N-NP}
@@ -291,29 +289,48 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_fwd(
 //   R0 aliases of m' survive), i.e. R0 NP-point FFTs per line; the forward
 //   runs the same steps in reverse and accumulates Z_m (m <= L) and Z_{N-m}.
 // The NP-point FFTs are Stockham radix-8 passes (radix 4 / 2 to finish) with
-// eight points per thread in registers and padded LDS between passes; input
-// and output rows are staged through LDS (row stride 17 doubles).
-constexpr int kStageStride = 17;  // doubles per staged row of 16 lines
+// eight points per thread in registers and padded LDS between passes.
+//
+// A work-group owns NC complex lines (2 NC consecutive real lines packed in
+// pairs), T = NP/8 threads per complex line, so a row of the line-contiguous
+// layouts is 16 NC contiguous bytes.  NC = 4: 64-byte half rows and about
+// 72 KiB of LDS, so two work-groups share a CU and one's copies and stores
+// overlap the other's passes.  The two halves of a 128-byte row belong to
+// neighbouring logical blocks, which xcd_block keeps on one XCD (one L2)
+// at about the same time.
+// The backward runs NC = 4 (launch bounds: at least two waves per SIMD, i.e.
+// at most 256 VGPRs).  The forward keeps NC = 8, one work-group per CU, and
+// overlaps instead the copy of the next phi decimation with the passes of
+// the current one inside the work-group; its alias twiddles come from a
+// full N-entry table (composing them from the NP roots, as the backward
+// does, would not fit its register budget).
+constexpr int kFftNCb = 4, kFftNCf = 8;
 
 __device__ __forceinline__ int fpad(int i) { return i + (i >> 3); }
 
-// LDS layout of the split-form kernels, in double2 units: the twiddle table
-// (all N roots exp(-2 pi i k / N), copied from global memory at the start)
-// sits past every other area, so that the passes and the alias sums read no
-// global memory: a global load there would be counted with the copies or
-// stores in flight (vmcnt completes in issue order) and wait for them.
-//   backward: coefficient block (16 NP: 2 NP rows of 8 slots) or staging of
-//             256-order chunks (17 min(256, NP)), then pass / output areas
-//             (reused);
-//   forward:  pass area 8 (NP + NP/8), stage of NP rows x 8 slots.
-template <int NP>
+// LDS layout, in double2 units.  The twiddles (backward: NP roots of order
+// NP, then the R0 roots exp(-2 pi i b / N), b < R0; forward: all N roots
+// exp(-2 pi i k / N); copied from global memory at the start) sit past every
+// other area, so that the passes and the alias sums read no global memory: a
+// global load there would be counted with the copies or stores in flight
+// (vmcnt completes in issue order) and wait for them.
+//   backward: coefficient block (2 NP rows of NC slots) or staging of
+//             256-order chunks (rows of 2 NC + 1 doubles), then the pass
+//             areas and the output staging (NC + 1 slots per row), reused;
+//   forward:  pass areas NC (NP + NP/8), then a stage of NP rows x NC slots;
+//             the output staging (rows of 2 NC + 1 doubles) reuses both.
+template <int NP, int NC>
 constexpr int fft3_tw_off_bwd() {
-  constexpr int a = 16 * NP, b = 17 * (NP < 256 ? NP : 256), c = 8 * (NP + NP / 8), d = 9 * NP;
+  constexpr int a = 2 * NP * NC, b = (2 * NC + 1) * (NP < 256 ? NP : 256);
+  constexpr int c = NC * (NP + NP / 8), d = (NC + 1) * NP;
   constexpr int ab = a > b ? a : b, cd = c > d ? c : d;
   return ab > cd ? ab : cd;
 }
-template <int NP>
-constexpr int fft3_tw_off_fwd() { return 8 * (NP + NP / 8) + 8 * NP; }
+template <int NP, int NC>
+constexpr int fft3_tw_off_fwd() {
+  constexpr int a = NC * (NP + NP / 8) + NC * NP, b = (2 * NC + 1) * (NP < 256 ? NP : 256);
+  return a > b ? a : b;
+}
 
 // The pass area of a complex line belongs to the wave that transforms it
 // (T = NP/8 <= 64 threads per line, lines never straddle a wave), and one
@@ -326,12 +343,11 @@ __device__ __forceinline__ void wave_lds_order() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// tw: roots of unity of order NP at stride ts (the plan's N-point table in
-// LDS with ts = R0)
+// tw: roots of unity in LDS, of order NP at stride ts (the NP-entry table
+// with ts = 1, or the N-entry table with ts = R0)
 template <int NP, int Ns>
 __device__ __forceinline__ void f3_passes(double2 (&v)[8], double2 *ln, int j,
-                                          const double2 *__restrict__ tw, int N, int ts,
-                                          double s) {
+                                          const double2 *__restrict__ tw, int ts, double s) {
   constexpr int T = NP / 8;
   constexpr int R = (NP / Ns >= 8) ? 8 : NP / Ns;
 #pragma unroll
@@ -363,7 +379,7 @@ __device__ __forceinline__ void f3_passes(double2 (&v)[8], double2 *ln, int j,
 #pragma unroll
       for (int r = 0; r < R2; r++) v[i * R2 + r] = ln[fpad(j + T * i + (NP / R2) * r)];
     wave_lds_order();
-    f3_passes<NP, Ns * R>(v, ln, j, tw, N, ts, s);
+    f3_passes<NP, Ns * R>(v, ln, j, tw, ts, s);
   }
 }
 
@@ -390,6 +406,15 @@ __device__ __forceinline__ int alias_hi(int m, int j1, int N, int NP) {
   return j1 == 0 ? 0 : m * j1 + N - NP * j1;
 }
 
+// exp(-2 pi i k / N), k < N = R0 NP, as exp(-2 pi i a / NP) exp(-2 pi i b / N)
+// with k = R0 a + b; a = k / R0 by a multiply-high with r0m = ceil(2^32 / R0)
+// (exact for k < 2^32 / R0; r0m is unused for R0 = 1)
+__device__ __forceinline__ double2 tw_n(const double2 *twp, const double2 *twr, int k, int R0,
+                                        unsigned r0m) {
+  const int a = R0 == 1 ? k : (int)__umulhi((unsigned)k, r0m);
+  return cmul(twp[a], twr[k - a * R0]);
+}
+
 // LDS DMA of 16 bytes per lane (global_load_lds_dwordx4) from a wave-uniform
 // base plus a 32-bit lane offset, as inline assembly: the compiler would wait
 // for a builtin DMA before every later LDS access, which would serialise the
@@ -410,19 +435,21 @@ __device__ __forceinline__ double2 cfma(double2 a, double2 w, double2 acc) {
   return make_double2(acc.x + a.x * w.x - a.y * w.y, acc.y + a.x * w.y + a.y * w.x);
 }
 
-template <int NP>
-__global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long nline,
-                                                 const double *__restrict__ X, FourierAddr xa,
-                                                 double *__restrict__ v,
-                                                 const double2 *__restrict__ twn) {
+template <int NP, int NC>
+__global__ void __launch_bounds__(NC * NP / 8, 2) k_fft3_bwd(int N, int R0, unsigned r0m, int L,
+                                                          long long nline,
+                                                          const double *__restrict__ X,
+                                                          FourierAddr xa, double *__restrict__ v,
+                                                          const double2 *__restrict__ twn) {
   extern __shared__ double2 lds[];
-  constexpr int T = NP / 8;
+  constexpr int T = NP / 8, NT = NC * T, W = 2 * NC, SS = W + 1;
   const int tid = threadIdx.x, cl = tid / T, j = tid % T;
   const int nd = blockIdx.y;
-  const long long line0 = (long long)blockIdx.x * 16;
+  const long long line0 = (long long)xcd_block(blockIdx.x, gridDim.x) * W;
   double *stage = reinterpret_cast<double *>(lds);
-  double2 *twl = lds + fft3_tw_off_bwd<NP>();
-  for (int i = tid; i < N; i += NP) twl[i] = twn[i];  // read after the staging barrier
+  double2 *twp = lds + fft3_tw_off_bwd<NP, NC>(), *twr = twp + NP;
+  // read after the staging barrier below
+  for (int i = tid; i < NP + R0; i += NT) twp[i] = twn[i < NP ? R0 * i : i - NP];
   double2 zlo[8], zhi[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) zlo[q] = zhi[q] = make_double2(0.0, 0.0);
@@ -430,19 +457,18 @@ __global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long
   if (xa.s_line == 1 && (nline & 1) == 0 && (xa.s_nd & 1) == 0 && (xa.q_stride & 1) == 0 &&
       ((unsigned long long)X & 15) == 0) {
     // Lines contiguous, rows 16-byte aligned (even strides, aligned base):
-    // the whole coefficient block
-    // (rows q = 0 .. 2L, 16 lines of 8 bytes each) goes straight to LDS with
-    // global_load_lds_dwordx4, one wait for all of it.  Row r sits at 128 r
-    // bytes; its 16-byte slot u holds lines 2 (u ^ sw(r)) and +1, sw(r) =
-    // (r >> 1) & 7, so the reads below (rows 2f - 1, 2f of consecutive
-    // orders f) fall in distinct slots.  Rows are padded to 2 NP; lines past
-    // the end read a valid pair (their results are never stored).
+    // the whole coefficient block (rows q = 0 .. 2L of the W lines) goes
+    // straight to LDS with global_load_lds_dwordx4, one wait for all of it.
+    // Row r's 16-byte slot u holds lines 2 (u ^ sw(r)) and +1, sw(r) =
+    // (r >> 1) & (NC - 1), so the reads below (rows 2f - 1, 2f of
+    // consecutive orders f) spread over the slots.  Rows are padded to 2 NP;
+    // lines past the end read a valid pair (their results are never stored).
     double2 *st2 = lds;
     const int nrow = 2 * L + 1;
-    for (int i0 = tid & ~63; i0 < 16 * NP; i0 += NP) {
+    for (int i0 = tid & ~63; i0 < 2 * NP * NC; i0 += NT) {
       const int i = i0 + (tid & 63);
-      const int r = i >> 3, u = i & 7;
-      const int c = u ^ ((r >> 1) & 7);
+      const int r = i / NC, u = i % NC;
+      const int c = u ^ ((r >> 1) & (NC - 1));
       long long line = line0 + 2 * c;
       if (line + 1 >= nline) line = line0;
       const double *src = X + q_offset(xa, r < nrow ? r : 0) + xbase + line;
@@ -452,7 +478,7 @@ __global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
-    auto row2 = [&](int r) { return st2[r * 8 + (cl ^ ((r >> 1) & 7))]; };
+    auto row2 = [&](int r) { return st2[r * NC + (cl ^ ((r >> 1) & (NC - 1)))]; };
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const int f = j + T * q;  // order of Z_f; Z_{f+N-NP} comes from order NP - f
@@ -477,23 +503,23 @@ __global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long
     const int f1 = min(f0 + 256, L + 1);
     const int q0 = f0 == 0 ? 0 : 2 * f0 - 1, nrow = 2 * (f1 - 1) - q0 + 1;
     // batches of 8 independent loads per thread, then the LDS stores
-    for (int base = 0; base < nrow * 16; base += 8 * NP) {
+    for (int base = 0; base < nrow * W; base += 8 * NT) {
       // unconditional loads (clamped to a valid element, zeroed after) so
       // the eight stay in flight together
       double val[8];
 #pragma unroll
       for (int u = 0; u < 8; u++) {
-        const int idx = base + tid + u * NP;
-        const long long line = line0 + (idx & 15);
-        const bool ok = idx < nrow * 16 && line < nline;
-        const int row = ok ? idx >> 4 : 0;
+        const int idx = base + tid + u * NT;
+        const long long line = line0 + (idx % W);
+        const bool ok = idx < nrow * W && line < nline;
+        const int row = ok ? idx / W : 0;
         val[u] = X[q_offset(xa, q0 + row) + xbase + (ok ? line : line0) * xa.s_line];
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
-        const int idx = base + tid + u * NP;
-        if (idx < nrow * 16)
-          stage[(idx >> 4) * kStageStride + (idx & 15)] = line0 + (idx & 15) < nline ? val[u] : 0.0;
+        const int idx = base + tid + u * NT;
+        if (idx < nrow * W)
+          stage[(idx / W) * SS + (idx % W)] = line0 + (idx % W) < nline ? val[u] : 0.0;
       }
     }
     __syncthreads();
@@ -501,18 +527,18 @@ __global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long
     for (int q = 0; q < 8; q++) {
       const int f = j + T * q;  // order of Z_f; Z_{f+N-NP} comes from order NP - f
       if (f >= f0 && f < f1) {
-        const double *p = stage + ((f == 0 ? 0 : 2 * f - 1) - q0) * kStageStride + 2 * cl;
+        const double *p = stage + ((f == 0 ? 0 : 2 * f - 1) - q0) * SS + 2 * cl;
         if (f == 0) {
           zlo[q] = make_double2(p[0], p[1]);
         } else {
-          const double aa = p[0], ab = p[1], ba = p[kStageStride], bb = p[kStageStride + 1];
+          const double aa = p[0], ab = p[1], ba = p[SS], bb = p[SS + 1];
           zlo[q] = make_double2(0.5 * (aa + bb), 0.5 * (ab - ba));
         }
       }
       const int g = NP - f;
       if (g >= f0 && g < f1 && g >= 1) {
-        const double *p = stage + (2 * g - 1 - q0) * kStageStride + 2 * cl;
-        const double aa = p[0], ab = p[1], ba = p[kStageStride], bb = p[kStageStride + 1];
+        const double *p = stage + (2 * g - 1 - q0) * SS + 2 * cl;
+        const double aa = p[0], ab = p[1], ba = p[SS], bb = p[SS + 1];
         zhi[q] = make_double2(0.5 * (aa - bb), 0.5 * (ab + ba));
       }
     }
@@ -526,22 +552,22 @@ __global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const int m = j + T * q;
-      double2 t1 = twl[alias_lo(m, j1)];
-      double2 t2 = twl[alias_hi(m, j1, N, NP)];
+      double2 t1 = tw_n(twp, twr, alias_lo(m, j1), R0, r0m);
+      double2 t2 = tw_n(twp, twr, alias_hi(m, j1, N, NP), R0, r0m);
       t1.y = -t1.y;  // e^{+...}
       t2.y = -t2.y;
       w[q] = cfma(zhi[q], t2, cmul(zlo[q], t1));
     }
-    f3_passes<NP, 1>(w, ln, j, twl, N, R0, 1.0);
+    f3_passes<NP, 1>(w, ln, j, twp, 1, 1.0);
     __syncthreads();  // every wave done with its pass area (under the staging)
-    // rows phi = R0 (j + T q) + j1, staged as [j2][8 complex lines] (stride 9)
+    // rows phi = R0 (j + T q) + j1, staged as [j2][NC complex lines] (stride NC + 1)
 #pragma unroll
-    for (int q = 0; q < 8; q++) lds[(j + T * q) * 9 + cl] = w[f3_pos<NP>(q)];
+    for (int q = 0; q < 8; q++) lds[(j + T * q) * (NC + 1) + cl] = w[f3_pos<NP>(q)];
     __syncthreads();
-    for (int idx = tid; idx < NP * 8; idx += NP) {
-      const int j2 = idx >> 3, c = idx & 7;
+    for (int idx = tid; idx < NP * NC; idx += NT) {
+      const int j2 = idx / NC, c = idx % NC;
       const long long la = line0 + 2 * c;
-      const double2 z = lds[j2 * 9 + c];
+      const double2 z = lds[j2 * (NC + 1) + c];
       double *o = v + (long long)nd * nnod + (long long)(R0 * j2 + j1) * nline + la;
       if (la + 1 < nline) {
         if (vec2)
@@ -558,41 +584,41 @@ __global__ void __launch_bounds__(NP) k_fft3_bwd(int N, int R0, int L, long long
   }
 }
 
-template <int NP>
-__global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long nline,
-                                                 const double *__restrict__ v,
-                                                 double *__restrict__ X, FourierAddr xa,
-                                                 const double2 *__restrict__ twn) {
+template <int NP, int NC>
+__global__ void __launch_bounds__(NC * NP / 8) k_fft3_fwd(int N, int R0, int L,
+                                                          long long nline,
+                                                          const double *__restrict__ v,
+                                                          double *__restrict__ X, FourierAddr xa,
+                                                          const double2 *__restrict__ twn) {
   extern __shared__ double2 lds[];
-  constexpr int T = NP / 8;
+  constexpr int T = NP / 8, NT = NC * T, W = 2 * NC, SS = W + 1;
   const int tid = threadIdx.x, cl = tid / T, j = tid % T;
   const int nd = blockIdx.y;
-  const long long line0 = (long long)blockIdx.x * 16;
+  const long long line0 = (long long)xcd_block(blockIdx.x, gridDim.x) * W;
   const long long nnod = nline * N;
   const bool vec2 = (nline & 1) == 0;
   double2 zlo[8], zhi[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) zlo[q] = zhi[q] = make_double2(0.0, 0.0);
   double2 *ln = lds + cl * (NP + NP / 8);
-  double2 *twl = lds + fft3_tw_off_fwd<NP>();
-  for (int i = tid; i < N; i += NP) twl[i] = twn[i];  // read after the first barrier
+  double2 *twl = lds + fft3_tw_off_fwd<NP, NC>();
+  for (int i = tid; i < N; i += NT) twl[i] = twn[i];  // read after the first barrier below
   // DMA form (lines contiguous in pairs, aligned base, per-component byte
   // offsets within 32 bits): the rows of the next phi decimation j1 + 1 are
-  // copied global -> LDS while the FFT of j1 runs.  LDS: [pass area: 8 (NP +
-  // NP/8)][stage: NP rows x 8 slots][twiddles]; row r's slot u holds lines
-  // 2 (u ^ (r & 7)) and +1 (XOR swizzle: the 64 lanes of a complex line read
-  // 8 distinct slots per 8 rows).
+  // copied global -> LDS while the FFT of j1 runs, into a stage past the
+  // pass areas; row r's slot u holds lines 2 (u ^ (r & (NC - 1))) and +1
+  // (XOR swizzle: consecutive threads of a complex line read distinct slots).
   const bool dma = vec2 && (((unsigned long long)v & 15) == 0) && nnod * 8 < (1ll << 32);
   if (dma) {
-    double2 *st2 = lds + 8 * (NP + NP / 8);
+    double2 *st2 = lds + NC * (NP + NP / 8);
     const unsigned st2_byte =
         (unsigned)(size_t)(const __attribute__((address_space(3))) void *)lds +
-        16u * (8 * (NP + NP / 8));
+        16u * (NC * (NP + NP / 8));
     auto issue = [&](int jj) {
       const double *base = v + (long long)nd * nnod + (long long)jj * nline + line0;
 #pragma unroll
-      for (int i0 = tid & ~63; i0 < 8 * NP; i0 += NP) {
-        const int i = i0 + (tid & 63), r = i >> 3, u = i & 7, c = u ^ (r & 7);
+      for (int i0 = tid & ~63; i0 < NC * NP; i0 += NT) {
+        const int i = i0 + (tid & 63), r = i / NC, u = i % NC, c = u ^ (r & (NC - 1));
         const long long cc = line0 + 2 * c + 1 < nline ? 2 * c : 0;  // past the end: line0
         const unsigned off = (unsigned)(((long long)R0 * r * nline + cc) * 8);
         fft_dma16(base, off, st2_byte + 16u * (unsigned)i0);
@@ -606,11 +632,11 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
 #pragma unroll
       for (int q = 0; q < 8; q++) {
         const int r = j + T * q;
-        w[q] = st2[r * 8 + (cl ^ (r & 7))];
+        w[q] = st2[r * NC + (cl ^ (r & (NC - 1)))];
       }
       __syncthreads();
       if (j1 + 1 < R0) issue(j1 + 1);
-      f3_passes<NP, 1>(w, ln, j, twl, N, R0, -1.0);
+      f3_passes<NP, 1>(w, ln, j, twl, R0, -1.0);
 #pragma unroll
       for (int q = 0; q < 8; q++) {
         const int k = j + T * q;
@@ -631,7 +657,7 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
       double2 z[8];
 #pragma unroll
       for (int u = 0; u < 8; u++) {
-        const int idx = tid + u * NP, j2 = idx >> 3, c = idx & 7;
+        const int idx = tid + u * NT, j2 = idx / NC, c = idx % NC;
         const long long la = line0 + 2 * c;
         const double *row = v + (long long)nd * nnod + (long long)(R0 * j2 + j1) * nline;
         if (vec2) {
@@ -644,18 +670,18 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
       }
 #pragma unroll
       for (int u = 0; u < 8; u++) {
-        const int idx = tid + u * NP;
-        const long long la = line0 + 2 * (idx & 7);
-        lds[(idx >> 3) * 9 + (idx & 7)] =
+        const int idx = tid + u * NT;
+        const long long la = line0 + 2 * (idx % NC);
+        lds[(idx / NC) * (NC + 1) + (idx % NC)] =
             make_double2(la < nline ? z[u].x : 0.0, la + 1 < nline ? z[u].y : 0.0);
       }
     }
     __syncthreads();
     double2 w[8];
 #pragma unroll
-    for (int q = 0; q < 8; q++) w[q] = lds[(j + T * q) * 9 + cl];
+    for (int q = 0; q < 8; q++) w[q] = lds[(j + T * q) * (NC + 1) + cl];
     __syncthreads();
-    f3_passes<NP, 1>(w, ln, j, twl, N, R0, -1.0);
+    f3_passes<NP, 1>(w, ln, j, twl, R0, -1.0);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       const int k = j + T * q;
@@ -696,20 +722,20 @@ __global__ void __launch_bounds__(NP) k_fft3_fwd(int N, int R0, int L, long long
           p[0] = sn * xar;
           p[1] = sn * xbr;
         } else {
-          double *p = stage + (2 * m - 1 - q0) * kStageStride + 2 * cl;
+          double *p = stage + (2 * m - 1 - q0) * SS + 2 * cl;
           p[0] = tsn * xar;
           p[1] = tsn * xbr;
-          p[kStageStride] = -tsn * xai;
-          p[kStageStride + 1] = -tsn * xbi;
+          p[SS] = -tsn * xai;
+          p[SS + 1] = -tsn * xbi;
         }
       }
     }
     __syncthreads();
-    for (int idx = tid; idx < nrow * 16; idx += NP) {
-      const int row = idx >> 4, l16 = idx & 15;
-      const long long line = line0 + l16;
+    for (int idx = tid; idx < nrow * W; idx += NT) {
+      const int row = idx / W, lw = idx % W;
+      const long long line = line0 + lw;
       if (line < nline)
-        X[q_offset(xa, q0 + row) + xbase + line * xa.s_line] = stage[row * kStageStride + l16];
+        X[q_offset(xa, q0 + row) + xbase + line * xa.s_line] = stage[row * SS + lw];
     }
   }
 }
@@ -782,18 +808,19 @@ int fft_plan_init(sht_plan *p) {
     fp.r0 = n / np;
     int tw_b = 0, tw_f = 0;
     switch (np) {
-      case 32: tw_b = fft3_tw_off_bwd<32>(); tw_f = fft3_tw_off_fwd<32>(); break;
-      case 64: tw_b = fft3_tw_off_bwd<64>(); tw_f = fft3_tw_off_fwd<64>(); break;
-      case 128: tw_b = fft3_tw_off_bwd<128>(); tw_f = fft3_tw_off_fwd<128>(); break;
-      case 256: tw_b = fft3_tw_off_bwd<256>(); tw_f = fft3_tw_off_fwd<256>(); break;
-      default: tw_b = fft3_tw_off_bwd<512>(); tw_f = fft3_tw_off_fwd<512>(); break;
+      case 32: tw_b = fft3_tw_off_bwd<32, kFftNCb>(); tw_f = fft3_tw_off_fwd<32, kFftNCf>(); break;
+      case 64: tw_b = fft3_tw_off_bwd<64, kFftNCb>(); tw_f = fft3_tw_off_fwd<64, kFftNCf>(); break;
+      case 128: tw_b = fft3_tw_off_bwd<128, kFftNCb>(); tw_f = fft3_tw_off_fwd<128, kFftNCf>(); break;
+      case 256: tw_b = fft3_tw_off_bwd<256, kFftNCb>(); tw_f = fft3_tw_off_fwd<256, kFftNCf>(); break;
+      default: tw_b = fft3_tw_off_bwd<512, kFftNCb>(); tw_f = fft3_tw_off_fwd<512, kFftNCf>(); break;
     }
-    fp.lds3b = (int)sizeof(double2) * (tw_b + n);
+    fp.lds3b = (int)sizeof(double2) * (tw_b + np + fp.r0);
     fp.lds3 = (int)sizeof(double2) * (tw_f + n);
+    fp.r0m = fp.r0 == 1 ? 0u : (unsigned)(((1ull << 32) + (unsigned)fp.r0 - 1) / (unsigned)fp.r0);
     int dev = 0, lds_max = 0;
     KSHT_HIP(hipGetDevice(&dev));
     KSHT_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    if (fp.lds3b > lds_max || fp.lds3 > lds_max) fp.fast = 0;  // table does not fit: generic form
+    if (fp.lds3b > lds_max || fp.lds3 > lds_max) fp.fast = 0;  // does not fit: generic form
   }
   if (fp.fast) {
     const int np = fp.np;
@@ -807,9 +834,9 @@ int fft_plan_init(sht_plan *p) {
     p->device_bytes += (long long)(sizeof(double2) * n);
 #define KSHT_FFT3_ATTR(NP)                                                                   \
   case NP:                                                                                   \
-    KSHT_HIP(hipFuncSetAttribute((const void *)k_fft3_bwd<NP>,                               \
+    KSHT_HIP(hipFuncSetAttribute((const void *)k_fft3_bwd<NP, kFftNCb>,                      \
                                  hipFuncAttributeMaxDynamicSharedMemorySize, fp.lds3b));     \
-    KSHT_HIP(hipFuncSetAttribute((const void *)k_fft3_fwd<NP>,                               \
+    KSHT_HIP(hipFuncSetAttribute((const void *)k_fft3_fwd<NP, kFftNCf>,                      \
                                  hipFuncAttributeMaxDynamicSharedMemorySize, fp.lds3));      \
     break;
     switch (np) {
@@ -825,15 +852,15 @@ int fft_plan_init(sht_plan *p) {
   return SHT_OK;
 }
 
-#define KSHT_FFT3_LAUNCH(K, LDS, ...)                                                        \
+#define KSHT_FFT3_LAUNCH(K, NC, LDS, ...)                                                    \
   do {                                                                                     \
-    const dim3 g3((unsigned)((nline + 15) / 16), (unsigned)ncomp);                         \
+    const dim3 g3((unsigned)((nline + 2 * (NC) - 1) / (2 * (NC))), (unsigned)ncomp);       \
     switch (p->fft.np) {                                                                   \
-      case 32: hipLaunchKernelGGL((K<32>), g3, dim3(32), (LDS), p->stream, __VA_ARGS__); break; \
-      case 64: hipLaunchKernelGGL((K<64>), g3, dim3(64), (LDS), p->stream, __VA_ARGS__); break; \
-      case 128: hipLaunchKernelGGL((K<128>), g3, dim3(128), (LDS), p->stream, __VA_ARGS__); break; \
-      case 256: hipLaunchKernelGGL((K<256>), g3, dim3(256), (LDS), p->stream, __VA_ARGS__); break; \
-      default: hipLaunchKernelGGL((K<512>), g3, dim3(512), (LDS), p->stream, __VA_ARGS__); break; \
+      case 32: hipLaunchKernelGGL((K<32, NC>), g3, dim3(4 * (NC)), (LDS), p->stream, __VA_ARGS__); break; \
+      case 64: hipLaunchKernelGGL((K<64, NC>), g3, dim3(8 * (NC)), (LDS), p->stream, __VA_ARGS__); break; \
+      case 128: hipLaunchKernelGGL((K<128, NC>), g3, dim3(16 * (NC)), (LDS), p->stream, __VA_ARGS__); break; \
+      case 256: hipLaunchKernelGGL((K<256, NC>), g3, dim3(32 * (NC)), (LDS), p->stream, __VA_ARGS__); break; \
+      default: hipLaunchKernelGGL((K<512, NC>), g3, dim3(64 * (NC)), (LDS), p->stream, __VA_ARGS__); break; \
     }                                                                                      \
   } while (0)
 
@@ -857,7 +884,8 @@ int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, doub
     }                                                                                      \
   } while (0)
   if (p->fft.fast)
-    KSHT_FFT3_LAUNCH(k_fft3_bwd, p->fft.lds3b, p->nphi, p->fft.r0, p->L, nline, X, xa, v, p->t.twn);
+    KSHT_FFT3_LAUNCH(k_fft3_bwd, kFftNCb, p->fft.lds3b, p->nphi, p->fft.r0, p->fft.r0m, p->L, nline, X,
+                     xa, v, p->t.twn);
   else
     KSHT_FFT_LAUNCH(k_fft_bwd, p->fft, p->L, nline, X, xa, v, p->t.tw);
   KSHT_HIP(hipGetLastError());
@@ -870,7 +898,8 @@ int launch_fft_fwd(sht_plan *p, int ncomp, const double *v, double *X, FourierAd
   const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);
   const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.n;
   if (p->fft.fast)
-    KSHT_FFT3_LAUNCH(k_fft3_fwd, p->fft.lds3, p->nphi, p->fft.r0, p->L, nline, v, X, xa, p->t.twn);
+    KSHT_FFT3_LAUNCH(k_fft3_fwd, kFftNCf, p->fft.lds3, p->nphi, p->fft.r0, p->L, nline, v, X, xa,
+                     p->t.twn);
   else
     KSHT_FFT_LAUNCH(k_fft_fwd, p->fft, p->L, nline, v, X, xa, p->t.tw);
   KSHT_HIP(hipGetLastError());

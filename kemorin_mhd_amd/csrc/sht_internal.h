@@ -42,8 +42,9 @@ struct FftPass {
 struct FftPlan {
   int n, npass, lines_per_wg, pad;  // pad: LDS elements per work-group
   FftPass pass[kMaxPass];
-  // split form n = r0 * np (np a power of two above L): 16 lines per work-group
+  // split form n = r0 * np (np a power of two above L)
   int fast, np, r0, lds3, lds3b;  // LDS bytes: forward, backward (whole-block staging)
+  unsigned r0m;                   // ceil(2^32 / r0): division by r0 as a multiply-high
 };
 
 // Addressing of an (order m, colatitude t, radius k, component nd) element
