@@ -279,6 +279,30 @@ __global__ void __launch_bounds__(kSliceThreads) k_bslice(
     const int d0 = 2 * i0, nd_run = min(2 * ni, n - d0) * ncomp;
     const int nall = nrad * nd_run;
     __syncthreads();
+    if (sa.s_nd == 1 && sa.s_j == ncomp) {
+      // components contiguous within a degree's record: a radius's run is
+      // nd_run consecutive doubles, staged radius by radius without any
+      // index division
+      for (int rr = 0; rr < nrad; rr++) {
+        const int kr = min(k0 + rr, nr - 1);  // radii >= nr read radius nr - 1
+        const double *rs = src + (long long)kr * sa.s_k + (long long)d0 * ncomp;
+        double *rd = run + rr * 128 * ncomp;
+        const bool live = k0 + rr < nr;
+        for (int base = 0; base < nd_run; base += 8 * kSliceThreads) {
+          double val[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int e = base + tid + u * kSliceThreads;
+            val[u] = rs[e < nd_run ? e : 0];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int e = base + tid + u * kSliceThreads;
+            if (e < nd_run) rd[e] = live ? val[u] : 0.0;
+          }
+        }
+      }
+    } else
     for (int base = 0; base < nall; base += 8 * kSliceThreads) {
       // independent loads first, then the LDS stores; the loads are
       // unconditional (clamped to a valid element, masked at the store) so
