@@ -47,6 +47,22 @@ struct FftPlan {
   unsigned r0m;                   // ceil(2^32 / r0): division by r0 as a multiply-high
 };
 
+// Work-group order of the GEMMs.  The hardware deals work-groups round-robin
+// to the 8 XCDs, each with its own L2, and dispatches them in order, so an
+// XCD with more work than the others holds up the whole grid.  Logical ids
+// are grouped in runs of G work-groups that share operands through L2; run r
+// goes to XCD r % 8, so each L2 sees whole runs while the 8 XCDs work
+// through neighbouring orders at the same time (the work per order falls
+// with M, so contiguous M ranges per XCD would be unbalanced; consecutive
+// runs must be consecutive orders for the same reason).  Ids past the last
+// full set of 8 runs keep their value.
+__device__ __forceinline__ long long xcd_group_order(long long b, long long n, long long G) {
+  const long long sup = 8 * G, nsup = n / sup;
+  if (b >= nsup * sup) return b;
+  const long long x = b % 8, i = b / 8;
+  return ((i / G) * 8 + x) * G + i % G;
+}
+
 // Addressing of an (order m, colatitude t, radius k, component nd) element
 // of a Legendre-side buffer:
 //   slot_off[m + L]*slot_mul + nd*s_nd + t*s_t + k*s_k.

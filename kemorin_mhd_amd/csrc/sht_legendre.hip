@@ -62,7 +62,10 @@ __global__ void __launch_bounds__(256) k_leg_bwd(
   constexpr int SBUF = 2 * NC * 2 * 256;
   __shared__ double sB[2][SBUF];  // [buf][sig][c][par][ii][kk]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  long long b = blockIdx.x;
+  // logical id = ((M n_rt + rt) ncg + cgl) nblk + tb in runs of a whole
+  // order per XCD (xcd_group_order): the theta blocks that read the same
+  // spectra are resident together on one L2
+  long long b = xcd_group_order(blockIdx.x, gridDim.x, (long long)d.n_rt * d.ncg * d.nblk);
   const int tb = (int)(b % d.nblk);
   b /= d.nblk;
   const int cgl = (int)(b % d.ncg);
@@ -264,13 +267,18 @@ __global__ void __launch_bounds__(256) k_leg_fwd(
   constexpr int SBUF = 2 * NC * 2 * 256;
   __shared__ double sA[2][SBUF];  // [buf][sig][f][sym/asym][tt][kk]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  long long b = blockIdx.x;
-  const int ib = (int)(b % d.nblk);
-  b /= d.nblk;
+  // logical id = ((ib (L+1) + M) n_rt + rt) ncg + cgl in runs of one (M, row
+  // block) per XCD (xcd_group_order), sharing the P rows on one L2; the run
+  // index ib (L+1) + M puts consecutive orders on consecutive XCDs (row
+  // blocks past the degrees of M are empty, and the work of row block ib
+  // would otherwise all land on two XCDs)
+  long long b = xcd_group_order(blockIdx.x, gridDim.x, (long long)d.n_rt * d.ncg);
   const int cgl = (int)(b % d.ncg);
   b /= d.ncg;
   const int rt = (int)(b % d.n_rt);
-  const int M = (int)(b / d.n_rt);
+  b /= d.n_rt;
+  const int M = (int)(b % (d.L + 1));
+  const int ib = (int)(b / (d.L + 1));
   const int L = d.L;
   const int ip0 = ipad[2 * M], ip1 = ipad[2 * M + 1];
   if (ib * 64 >= ip0) return;  // whole work-group beyond the degrees of M

@@ -471,22 +471,6 @@ __global__ void __launch_bounds__(kSliceThreads) k_fslice(
 // next step's copies in flight before the A slices of this step are used.
 constexpr int kGemmThreads = 512;
 
-// Work-group order of the GEMMs.  The hardware deals work-groups round-robin
-// to the 8 XCDs, each with its own L2, and dispatches them in order, so an
-// XCD with more work than the others holds up the whole grid.  Logical ids
-// are grouped in runs of G work-groups that share operands through L2; run r
-// goes to XCD r % 8, so each L2 sees whole runs while the 8 XCDs work
-// through neighbouring orders at the same time (the work per order falls
-// with M, so contiguous M ranges per XCD would be unbalanced; consecutive
-// runs must be consecutive orders for the same reason).  Ids past the last
-// full set of 8 runs keep their value.
-__device__ __forceinline__ long long xcd_group_order(long long b, long long n, long long G) {
-  const long long sup = 8 * G, nsup = n / sup;
-  if (b >= nsup * sup) return b;
-  const long long x = b % 8, i = b / 8;
-  return ((i / G) * 8 + x) * G + i % G;
-}
-
 __device__ __forceinline__ unsigned lds_addr(const void *p) {
   return (unsigned)(size_t)(const __attribute__((address_space(3))) void *)p;
 }
