@@ -51,7 +51,86 @@ def parse():
     ap.add_argument("--cpu-radii", type=int, default=16, help="radii in the CPU sample")
     ap.add_argument("--copy-calib", type=float, default=0.0,
                     help="only run a device copy of this many GiB (PMC calibration)")
+    ap.add_argument("--shells", type=int, default=0,
+                    help="only run the batched-shell workload: this many seeded uneven shells, "
+                         "one HIP graph launch against a loop of per-shell calls")
     return ap.parse_args()
+
+
+def run_shells(args, torch):
+    """Batched uneven shells: L drawn from the odd values in 63..383 (Ntheta =
+    3(L+1)/2, Nphi = 2 Ntheta), Nr from 33..129, one vector + one scalar per
+    shell in both directions.  A step is one backward and one forward
+    transform of every shell: as a loop of per-shell calls on one stream, and
+    as one launch of the shells' captured graph."""
+    from kemorin_mhd_amd import (SHT_PREC_FP64, SHT_PREC_SPLIT_INT8, ShellBatch, ShellConfig,
+                                 SphTransPlan)
+    rng = np.random.default_rng(20261202)
+    Ls = [int(x) for x in rng.choice(np.arange(63, 384, 2), size=args.shells)]
+    nrs = [int(x) for x in rng.integers(33, 130, size=args.shells)]
+    precs = ["fp64", "split"] if args.precision in ("auto", "both") else [args.precision]
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(20261202)
+    flops = sum(flops_per_contraction(dict(L=L, nr=nr, nth=3 * (L + 1) // 2)) * (6 + 6)
+                for L, nr in zip(Ls, nrs))
+    out = {"metric": "batched uneven shells: all-shell round trips/s, one HIP graph vs a loop "
+                     "of per-shell calls", "unit": "round_trips/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+           "data": "synthetic (seeded Gaussian spectra and grids)",
+           "config": {"workload": f"{args.shells} shells, L odd in 63..383, Nr in 33..129, "
+                                  "Ntheta=3(L+1)/2, Nphi=2 Ntheta, 1 vector + 1 scalar each way",
+                      "L": Ls, "nr": nrs},
+           "fp64_equivalent_flop_per_step": flops, "engines": {}}
+    stream = torch.cuda.Stream()
+    for prec in precs:
+        plans, bufs = [], []
+        with torch.cuda.stream(stream):
+            for L, nr in zip(Ls, nrs):
+                c = ShellConfig(L, nr, 3 * (L + 1) // 2, 3 * (L + 1))
+                p = SphTransPlan(c, max_ncomp=4, precision=SHT_PREC_SPLIT_INT8 if prec == "split"
+                                 else SHT_PREC_FP64, n_slices=args.slices)
+                p.set_stream(stream.cuda_stream)
+                ws = torch.randn(c.n_rj(4), generator=gen, dtype=torch.float64, device="cuda")
+                vin = torch.randn(c.n_rtp(4), generator=gen, dtype=torch.float64, device="cuda")
+                vout = torch.empty(c.n_rtp(4), dtype=torch.float64, device="cuda")
+                wr = torch.empty(c.n_rj(4), dtype=torch.float64, device="cuda")
+                plans.append(p)
+                bufs.append((ws, vout, vin, wr))
+        stream.synchronize()
+        plan_bytes = sum(p.device_bytes() for p in plans)
+
+        def loop():
+            for p, (ws, vout, vin, wr) in zip(plans, bufs):
+                p.sph_backward_transforms(1, 1, ws, vout)
+                p.sph_forward_transforms(1, 1, vin, wr)
+
+        batch = ShellBatch([(p, (1, 1), ws, vout, (1, 1), vin, wr)
+                            for p, (ws, vout, vin, wr) in zip(plans, bufs)])
+        res = {}
+        for name, fn in (("loop", loop), ("graph", lambda: batch.launch(stream.cuda_stream))):
+            for _ in range(args.warmup):
+                fn()
+            stream.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                fn()
+            stream.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / args.steps
+            res[name] = {"ms_per_step": ms, "round_trips_per_s": 1e3 / ms,
+                         "gflops_fp64_equivalent": flops / ms / 1e6}
+        res["graph_speedup_vs_loop"] = res["loop"]["ms_per_step"] / res["graph"]["ms_per_step"]
+        res["plan_device_bytes"] = plan_bytes
+        res["buffer_bytes"] = sum(8 * sum(t.numel() for t in b) for b in bufs)
+        out["engines"][prec] = res
+        batch.close()
+        for p in plans:
+            p.close()
+        del bufs
+        torch.cuda.empty_cache()
+    main_prec = precs[-1]
+    out["value"] = out["engines"][main_prec]["graph"]["round_trips_per_s"]
+    out["engine"] = main_prec
+    print(json.dumps(out))
 
 
 def flops_per_contraction(c):
@@ -284,8 +363,9 @@ def cpu_baseline(cfg_d, nr_sample):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; shard radii across processes "
-                          "(see DESIGN.md) -- run with --gpus 1"}))
+        print(json.dumps({"error": "this engine runs on one GPU (the transform has no coupling "
+                          "across radius: run one process per GPU on its share of the radii) "
+                          "-- run with --gpus 1"}))
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():
@@ -299,6 +379,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calib_bytes_each_way": 8 * n, "launches": 5}))
+        return
+    if args.shells > 0:
+        run_shells(args, torch)
         return
     cfg_d = CONFIGS[args.config]
     from kemorin_mhd_amd import sht as S

@@ -35,7 +35,7 @@ extern "C" {
 enum {
   SHT_OK = 0,
   SHT_ERR_ARG = 1,         /* bad descriptor, size or null pointer   */
-  SHT_ERR_UNSUPPORTED = 2, /* e.g. FFT length with a prime factor > 16 */
+  SHT_ERR_UNSUPPORTED = 2, /* e.g. FFT length beyond the LDS capacity   */
   SHT_ERR_OOM = 3,         /* device allocation failed               */
   SHT_ERR_HIP = 4,         /* HIP runtime error                      */
   SHT_ERR_STATE = 5        /* plan not ready / wrong precision mode  */
@@ -130,6 +130,34 @@ int sht_exchange_rtp_to_rtm(sht_plan *plan, int ncomp, const double *d_rtp,
 int sht_exchange_gather_rev(sht_plan *plan, int ncomp, long long n_dst,
                             long long n_src, const int *d_irev,
                             const double *d_src, double *d_dst);
+
+/* Shell batch: many independent shells (plans of differing L, Nr and grid,
+ * each with its own device buffers) captured once into one HIP graph, one
+ * backward and one forward transform per shell, and replayed with a single
+ * launch.  No counterpart in the reference, which transforms one shell per
+ * call (SS/t_sph_transforms.f90:74,250); it is for callers that carry many
+ * uneven shells and would otherwise pay one launch sequence per shell with
+ * most of the device idle.  Shells are dealt to parallel graph branches.
+ * A field set of (0, 0) skips that direction.  The plans' stage timers must
+ * be off; the graph keeps the plans' workspaces and the buffer pointers, so
+ * neither may be freed, and the plans may not run other work concurrently
+ * with a launch.  Refill the inputs in place between launches. */
+typedef struct sht_batch sht_batch;
+typedef struct {
+  sht_plan *plan;
+  int nvector_b, nscalar_b;      /* backward field set                  */
+  const double *d_WS_rj;         /* backward input (spectral layout)    */
+  double *d_v_rtp;               /* backward output (grid fields)       */
+  int nvector_f, nscalar_f;      /* forward field set                   */
+  const double *d_v_in;          /* forward input (grid fields)         */
+  double *d_WR_rj;               /* forward output (spectral layout)    */
+} sht_shell_call;
+int sht_batch_create(const sht_shell_call *calls, int n, sht_batch **batch);
+/* Launch on hip_stream (NULL = the batch's own blocking stream). */
+int sht_batch_launch(sht_batch *batch, void *hip_stream);
+/* Wait for the launches on the batch's own stream. */
+int sht_batch_synchronize(sht_batch *batch);
+int sht_batch_destroy(sht_batch *batch);
 
 /* Diagnostics: checks the MFMA lane layouts this library assumes on the
  * current device (FP64 16x16x4 and INT8 16x16x64).  0 = as assumed. */

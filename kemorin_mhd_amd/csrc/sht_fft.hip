@@ -6,7 +6,7 @@
 //   backward x_j = a0 + sum_{m>=1} (a_m cos + b_m sin)   (m <= L only).
 // Two real lines are packed as the real and imaginary parts of one complex
 // line; a work-group runs several complex Stockham FFTs (mixed radix 8/4/2/3/
-// 5 and a direct DFT for other primes up to 13) in LDS.  The Fourier side
+// 5, small DFTs for 7, 11, 13 and direct sums for larger primes) in LDS.  The Fourier side
 // holds orders m <= L in the compact order q = 0 (a0), 2m-1 (a_m), 2m (b_m);
 // the grid side is the reference's field-major v_rtp with phi slowest.
 #include <algorithm>
@@ -135,6 +135,67 @@ __device__ void fft_pass(double2 *buf, int n, int F, int Ns, const double2 *__re
   __syncthreads();
 }
 
+// One Stockham pass of a prime radix R above 13 (FFTPACK accepts any length;
+// Nphi = 3 (L+1) has factors up to L+1).  Too many points per butterfly for
+// registers: every output is a direct sum over its R inputs, written out of
+// place to scr (F n elements past buf) and copied back.  Output q of the
+// butterfly j = (j / Ns) Ns + k is sum_r x_r W^{r (k + Ns q)}, W = exp(s 2 pi
+// i / (Ns R)), the twiddle of the pass and the R-point DFT in one root; tb
+// holds exp(-2 pi i e / (Ns R)), e < Ns R.
+__device__ void fft_pass_big(double2 *buf, double2 *scr, int n, int F, int Ns, int R,
+                             const double2 *__restrict__ tb, double s) {
+  const int nR = n / R, NR = Ns * R, tot = F * n;
+  if (Ns == 1) {
+    // first pass (the plan orders these primes first): no twiddle, output q
+    // of butterfly j is sum_r x_{j + r nR} w^{r q}, w = exp(s 2 pi i / R).
+    // Consecutive lanes take consecutive q of one butterfly, so each input
+    // is one broadcast LDS read; the powers w^{r q} come from a recurrence,
+    // re-anchored on the exact table every 16 terms.
+    for (int o = threadIdx.x; o < tot; o += kFftThreads) {
+      const int f = o / n, pos = o - f * n;
+      const int j = pos / R, q = pos - j * R;
+      const double2 *src = buf + (long)f * n + j;
+      const double2 wq = make_double2(tb[q].x, -s * tb[q].y);
+      double2 acc = make_double2(0.0, 0.0), t = make_double2(1.0, 0.0);
+      int e = 0;  // r q mod R
+      for (int r = 0; r < R; r++) {
+        if ((r & 15) == 0) t = make_double2(tb[e].x, -s * tb[e].y);
+        const double2 x = src[r * nR];
+        acc.x += x.x * t.x - x.y * t.y;
+        acc.y += x.x * t.y + x.y * t.x;
+        t = cmul(t, wq);
+        e += q;
+        if (e >= R) e -= R;
+      }
+      scr[o] = acc;
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < tot; o += kFftThreads) buf[o] = scr[o];
+    __syncthreads();
+    return;
+  }
+  for (int o = threadIdx.x; o < tot; o += kFftThreads) {
+    const int f = o / n, pos = o - f * n;
+    const int jb = pos / NR, w = pos - jb * NR, q = w / Ns, k = w - q * Ns;
+    const double2 *src = buf + (long)f * n + jb * Ns + k;
+    const int e0 = k + Ns * q;
+    double2 acc = make_double2(0.0, 0.0);
+    int e = 0;
+    for (int r = 0; r < R; r++) {
+      const double2 t = tb[e];
+      const double2 x = src[r * nR];
+      acc.x += x.x * t.x + s * x.y * t.y;  // x * (t.x, -s t.y)
+      acc.y += x.y * t.x - s * x.x * t.y;
+      e += e0;
+      if (e >= NR) e -= NR;
+    }
+    scr[o] = acc;
+  }
+  __syncthreads();
+  for (int o = threadIdx.x; o < tot; o += kFftThreads) buf[o] = scr[o];
+  __syncthreads();
+}
+
 template <int E, bool GEN>
 __device__ void fft_run(double2 *buf, const FftPlan &fp, int F, const double2 *__restrict__ tw,
                         double s) {
@@ -147,14 +208,15 @@ __device__ void fft_run(double2 *buf, const FftPlan &fp, int F, const double2 *_
       case 2: fft_pass<E, 2>(buf, fp.n, F, ps.Ns, t, s); break;
       case 3: fft_pass<E, 3>(buf, fp.n, F, ps.Ns, t, s); break;
       default:
-        // radices 5, 7, 11, 13 only in the GEN instantiation (register cost)
+        // radices 5, 7, 11, 13 and the larger primes only in the GEN
+        // instantiation (register and LDS cost)
         if constexpr (GEN) {
           switch (ps.R) {
             case 5: fft_pass<E, 5>(buf, fp.n, F, ps.Ns, t, s); break;
             case 7: fft_pass<E, 7>(buf, fp.n, F, ps.Ns, t, s); break;
             case 11: fft_pass<E, 11>(buf, fp.n, F, ps.Ns, t, s); break;
             case 13: fft_pass<E, 13>(buf, fp.n, F, ps.Ns, t, s); break;
-            default: break;
+            default: fft_pass_big(buf, buf + (long)F * fp.n, fp.n, F, ps.Ns, ps.R, t, s); break;
           }
         }
         break;
@@ -752,14 +814,28 @@ int fft_plan_init(sht_plan *p) {
   while (rem % 2 == 0) { fac.push_back(2); rem /= 2; }
   for (int q : {3, 5, 7, 11, 13})
     while (rem % q == 0) { fac.push_back(q); rem /= q; }
-  if (rem != 1 || (int)fac.size() > kMaxPass) {
-    set_error("FFT length %d has a prime factor > 13 (unsupported)", n);
+  // primes above 13 (direct-sum passes through an LDS scratch copy), first,
+  // so that the first of them runs without twiddles
+  fp.big = 0;
+  std::vector<int> big;
+  for (int q = 17; rem > 1; q += 2) {
+    if ((long long)q * q > rem) q = rem;
+    while (rem % q == 0) {
+      big.push_back(q);
+      rem /= q;
+      fp.big = 1;
+    }
+  }
+  fac.insert(fac.begin(), big.begin(), big.end());
+  if ((int)fac.size() > kMaxPass) {
+    set_error("FFT length %d has more than %d factors (unsupported)", n, kMaxPass);
     return SHT_ERR_UNSUPPORTED;
   }
   const int E = n <= kFftElemsSmall ? kFftElemsSmall : kFftElemsLarge;
-  int F = E / n;  // complex FFTs per work-group
+  int F = E / (fp.big ? 2 * n : n);  // complex FFTs per work-group (plus the scratch copy)
   if (F < 1) {
-    set_error("FFT length %d exceeds the LDS capacity (%d)", n, kFftElemsLarge);
+    set_error("FFT length %d exceeds the LDS capacity (%d%s)", n, kFftElemsLarge,
+              fp.big ? " with a scratch copy for its prime factors above 13" : "");
     return SHT_ERR_UNSUPPORTED;
   }
   fp.pad = E;
@@ -775,16 +851,24 @@ int fft_plan_init(sht_plan *p) {
     ps.Ns = Ns;
     ps.tw_off = (int)tw.size();
     ps.pad = 0;
-    for (int k = 0; k < Ns; k++)
-      for (int r = 0; r < R; r++) {
-        const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)(k * r) /
-                              (long double)(Ns * R);
+    if (R > 13) {
+      // direct-sum pass: all Ns R roots of order Ns R
+      for (int e = 0; e < Ns * R; e++) {
+        const long double a =
+            -2.0L * 3.14159265358979323846264338327950288L * (long double)e / (long double)(Ns * R);
         tw.push_back(make_double2((double)cosl(a), (double)sinl(a)));
       }
+    } else
+      for (int k = 0; k < Ns; k++)
+        for (int r = 0; r < R; r++) {
+          const long double a = -2.0L * 3.14159265358979323846264338327950288L *
+                                (long double)(k * r) / (long double)(Ns * R);
+          tw.push_back(make_double2((double)cosl(a), (double)sinl(a)));
+        }
     fp.pass[fp.npass++] = ps;
     Ns *= R;
   }
-  const int lds = (int)(sizeof(double2) * (size_t)f2 * n);
+  const int lds = (int)(sizeof(double2) * (size_t)f2 * n * (fp.big ? 2 : 1));
   bool gen = false;
   for (int R : fac) gen |= !(R == 2 || R == 3 || R == 4 || R == 8);
   p->fft_generic = gen;
@@ -868,7 +952,7 @@ int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, doub
   const long long nline = (long long)p->nr * p->nth;
   const int B = p->fft.lines_per_wg;
   const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);
-  const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.n;
+  const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.n * (p->fft.big ? 2 : 1);
 #define KSHT_FFT_LAUNCH(K, ...)                                                              \
   do {                                                                                     \
     if (p->fft.pad == kFftElemsSmall) {                                                    \
@@ -896,7 +980,7 @@ int launch_fft_fwd(sht_plan *p, int ncomp, const double *v, double *X, FourierAd
   const long long nline = (long long)p->nr * p->nth;
   const int B = p->fft.lines_per_wg;
   const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);
-  const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.n;
+  const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.n * (p->fft.big ? 2 : 1);
   if (p->fft.fast)
     KSHT_FFT3_LAUNCH(k_fft3_fwd, kFftNCf, p->fft.lds3, p->nphi, p->fft.r0, p->L, nline, v, X, xa,
                      p->t.twn);

@@ -41,6 +41,7 @@ struct FftPass {
 };
 struct FftPlan {
   int n, npass, lines_per_wg, pad;  // pad: LDS elements per work-group
+  int big;                          // a prime factor above 13 (LDS scratch copy)
   FftPass pass[kMaxPass];
   // split form n = r0 * np (np a power of two above L)
   int fast, np, r0, lds3, lds3b;  // LDS bytes: forward, backward (whole-block staging)

@@ -46,6 +46,12 @@ class _Desc(ctypes.Structure):
                 ("n_slices", ctypes.c_int), ("flags", ctypes.c_int)]
 
 
+class _ShellCall(ctypes.Structure):
+    _fields_ = [("plan", _vp), ("nvector_b", ctypes.c_int), ("nscalar_b", ctypes.c_int),
+                ("d_WS_rj", _vp), ("d_v_rtp", _vp), ("nvector_f", ctypes.c_int),
+                ("nscalar_f", ctypes.c_int), ("d_v_in", _vp), ("d_WR_rj", _vp)]
+
+
 def lib() -> ctypes.CDLL:
     """Load the in-tree HIP library (fails loudly when it is not built)."""
     global _LIB
@@ -70,6 +76,11 @@ def lib() -> ctypes.CDLL:
         L.sht_plan_timer_enable.argtypes = [_vp, ctypes.c_int]
         L.sht_plan_timer_read.argtypes = [_vp, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+        L.sht_batch_create.argtypes = [ctypes.POINTER(_ShellCall), ctypes.c_int,
+                                       ctypes.POINTER(_vp)]
+        L.sht_batch_launch.argtypes = [_vp, _vp]
+        L.sht_batch_synchronize.argtypes = [_vp]
+        L.sht_batch_destroy.argtypes = [_vp]
         L.sht_get_last_error.argtypes = [ctypes.c_char_p, ctypes.c_int]
         L.sht_version.restype = ctypes.c_char_p
         _LIB = L
@@ -292,6 +303,48 @@ class SphTransPlan:
         self._bind(src, dst)
         _check(lib().sht_exchange_gather_rev(self._h, ncomp, n_dst, n_src, irev.data_ptr(),
                                              _ptr(src), _ptr(dst)), "sht_exchange_gather_rev")
+
+
+class ShellBatch:
+    """Many independent shells replayed as one HIP graph (sht_batch_create).
+
+    ``calls``: sequence of (plan, (nv_b, ns_b), WS, v_rtp, (nv_f, ns_f), v_in,
+    WR) with device tensors; a field set of (0, 0) skips that direction (its
+    buffers may be None).  The graph keeps the plans and the buffer
+    addresses: keep them alive, refill inputs in place between launches.
+    No counterpart in the reference, which transforms one shell per call
+    (SS/t_sph_transforms.f90:74,250)."""
+
+    def __init__(self, calls):
+        arr = (_ShellCall * len(calls))()
+        self._keep = []
+        for i, (plan, fb, WS, v, ff, vin, WR) in enumerate(calls):
+            self._keep.append((plan, WS, v, vin, WR))
+            arr[i] = _ShellCall(plan.handle, fb[0], fb[1], _ptr(WS) if WS is not None else None,
+                                _ptr(v) if v is not None else None, ff[0], ff[1],
+                                _ptr(vin) if vin is not None else None,
+                                _ptr(WR) if WR is not None else None)
+        h = _vp()
+        _check(lib().sht_batch_create(arr, len(calls), ctypes.byref(h)), "sht_batch_create")
+        self._h = h
+
+    def launch(self, stream_ptr: int | None = None):
+        """One replay of every shell's transforms (None: the batch's own stream)."""
+        _check(lib().sht_batch_launch(self._h, stream_ptr), "sht_batch_launch")
+
+    def synchronize(self):
+        _check(lib().sht_batch_synchronize(self._h), "sht_batch_synchronize")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().sht_batch_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def selftest() -> int:

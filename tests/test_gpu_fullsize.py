@@ -204,16 +204,18 @@ def test_cfg2_fp64_linearity(torch_cuda):
         t.cuda.empty_cache()
 
 
-def test_truncation_L1023(torch_cuda):
-    """Config 4's truncation L=1023 on the smallest grid it allows (Ntheta =
-    L+1 = 1024, Nphi = 2L+2 = 2048: past the split-form FFT, so the
-    mixed-radix FFT runs) and 2 radii, to keep the host's O(Ntheta^3) Gauss
-    points short: FP64 engine against the oracle (1e-12 per component),
-    split engine against the FP64 engine (1e-11), whole backward (2 vector +
-    2 scalar) and forward (3 vector)."""
+@pytest.mark.parametrize("nth,nphi", [(1024, 2048), (1536, 3072)])
+def test_truncation_L1023(torch_cuda, nth, nphi):
+    """Config 4's truncation L=1023, on the smallest grid it allows (Ntheta =
+    L+1 = 1024, Nphi = 2L+2 = 2048) and on config 4's own grid (Ntheta =
+    1536, Nphi = 3072); both are past the split-form FFT, so the mixed-radix
+    FFT runs.  2 radii keep the oracle short (the radii are independent):
+    FP64 engine against the oracle (1e-12 per component), split engine
+    against the FP64 engine (1e-11), whole backward (2 vector + 2 scalar) and
+    forward (3 vector)."""
     t = torch_cuda
-    cfg = ShellConfig(1023, 2, 1024, 2048)
-    _say("[gpu] L=1023 Ntheta=1024 Nphi=2048, 2 radii")
+    cfg = ShellConfig(1023, 2, nth, nphi)
+    _say(f"[gpu] L=1023 Ntheta={nth} Nphi={nphi}, 2 radii")
     orc = O.OraclePlan(cfg.L, cfg.nr, cfg.nth, cfg.nphi, cfg.radius(), full_tables=False)
     sp = spectra(cfg, 8, 401)
     spf = spectra(cfg, 9, 403)

@@ -17,6 +17,9 @@ CASES = {
     "L15_nr5_nth25_odd": ShellConfig(15, 5, 25, 40),
     "L63_nr17_nth96": ShellConfig(63, 17, 96, 192),
     "L20_nr2_nth33_nphi42": ShellConfig(20, 2, 33, 42),
+    # Nphi = 3 (L+1) with prime factors above 13 (direct-sum FFT passes)
+    "L37_nr3_nth57_nphi114": ShellConfig(37, 3, 57, 114),
+    "L45_nr2_nth69_nphi138": ShellConfig(45, 2, 69, 138),
 }
 TOL = 1e-12
 

@@ -65,6 +65,18 @@ def test_plan_create_rejects_bad_descriptors(bad):
     assert "sht_plan_create" in sht.last_error()
 
 
+def test_batch_create_rejects_bad_arguments():
+    h = ctypes.c_void_p()
+    L = sht.lib()
+    assert L.sht_batch_create(None, 1, ctypes.byref(h)) == 1 and not h.value
+    calls = (sht._ShellCall * 1)()
+    assert L.sht_batch_create(calls, 0, ctypes.byref(h)) == 1
+    assert L.sht_batch_create(calls, 1, ctypes.byref(h)) == 1  # no plan
+    assert "shell 0 has no plan" in sht.last_error()
+    assert L.sht_batch_launch(None, None) == 1
+    assert L.sht_batch_destroy(None) == 0
+
+
 def test_plan_create_rejects_null_and_nonpositive_radius():
     h = ctypes.c_void_p()
     assert sht.lib().sht_plan_create(None, ctypes.byref(h)) == 1
