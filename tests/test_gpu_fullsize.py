@@ -226,7 +226,7 @@ def test_truncation_L1023(torch_cuda, nth, nphi):
     outs = {}
     for prec in (SHT_PREC_FP64, SHT_PREC_SPLIT_INT8):
         plan = SphTransPlan(cfg, max_ncomp=9, precision=prec)
-        _say(f"    plan {prec} ready")
+        _say(f"    plan {prec} ready: {plan.device_bytes() / 1e9:.2f} GB on the device")
         try:
             v = t.empty(cfg.n_rtp(8), dtype=t.float64, device="cuda")
             s = t.empty(cfg.n_rj(9), dtype=t.float64, device="cuda")
