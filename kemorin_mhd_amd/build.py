@@ -43,14 +43,17 @@ def build(verbose: bool = False) -> str:
         return obj
 
     def waits():
-        # the split GEMMs count their own memory waits: verify the generated
-        # code keeps every in-flight load's registers untouched on all paths
+        # the split GEMMs count their own memory waits, and they and the
+        # split-form FFTs issue LDS DMA as inline assembly: verify the
+        # generated code keeps every in-flight load's registers untouched and
+        # the wait states the assembly needs, on all paths
         from .isa_waits import check_source
-        probs = check_source(cc, FLAGS, os.path.join(CSRC, "sht_split.hip"), "k_leg_")
-        bad = {k: v for k, v in probs.items() if v}
-        if not probs or bad:
-            raise RuntimeError("split GEMM wait check failed:\n" +
-                               "\n".join(f"{k}: {v[:5]}" for k, v in bad.items()))
+        for src, pattern in (("sht_split.hip", "k_leg_"), ("sht_fft.hip", "k_fft3_")):
+            probs = check_source(cc, FLAGS, os.path.join(CSRC, src), pattern)
+            bad = {k: v for k, v in probs.items() if v}
+            if not probs or bad:
+                raise RuntimeError(f"{src} wait check failed:\n" +
+                                   "\n".join(f"{k}: {v[:5]}" for k, v in bad.items()))
 
     with ThreadPoolExecutor(max_workers=min(7, os.cpu_count() or 1)) as ex:
         fut = ex.submit(waits)

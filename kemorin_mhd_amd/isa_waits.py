@@ -15,6 +15,9 @@ queues of in-flight vector-memory operations to a fixed point; every
 instruction that reads or writes a VGPR still owned by an in-flight load is
 reported, as is any scratch (private memory) traffic in the checked kernels.
 
+The same checks run on the split-form FFT kernels, whose LDS DMA is inline
+assembly too (their other loads are the compiler's own).
+
 The compiler's hazard recognizer does not look inside inline assembly either,
 so `check()` also verifies two gfx9 wait-state rules before every vector-
 memory instruction, along all paths: one wait state after an SALU write of M0
@@ -176,7 +179,10 @@ def _step(q: Tuple[frozenset, ...], s: str, bad: Set[str]) -> Tuple[frozenset, .
             src = _regs(",".join(parts[1:]))
         else:
             dst, src = frozenset(), _regs(args)
-        if any(d & (src | dst) for d in q):
+        # a later load may target the registers of an earlier one still in
+        # flight (loads return in issue order, so the later value lands last);
+        # reading them as an address or store value may not
+        if any(d & src for d in q):
             bad.add(s)
         return (q + (dst,))[-MAX_QUEUE:]
     used = _regs(args)

@@ -200,6 +200,19 @@ def test_wait_check_flags_use_and_copy_before_the_wait():
     assert len(_waits(reuse)) == 1
 
 
+def test_wait_check_load_order():
+    # a later load into the registers of one still in flight lands last
+    # (loads return in issue order): accepted; using them as an address is not
+    waw = ("\tglobal_load_dwordx4 v[2:5], v1, s[0:1]\n"
+           "\tglobal_load_dwordx4 v[4:7], v8, s[0:1]\n"
+           "\ts_waitcnt vmcnt(0)\n")
+    assert _waits(waw) == []
+    addr = ("\tglobal_load_dwordx2 v[2:3], v1, s[0:1]\n"
+            "\tglobal_load_dwordx4 v[4:7], v2, s[0:1]\n"
+            "\ts_waitcnt vmcnt(0)\n")
+    assert len(_waits(addr)) == 1
+
+
 def test_wait_check_follows_every_path():
     # the copy after the branch is skipped on one path, so vmcnt(1) does
     # not retire the register load there
