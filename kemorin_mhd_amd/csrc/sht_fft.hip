@@ -20,6 +20,8 @@ constexpr int kFftThreads = 256;
 // Complex elements per work-group held in LDS: 3072 (48 KiB, three
 // work-groups per CU) for Nphi <= 3072, 6144 (96 KiB) for longer lines.
 constexpr int kFftElemsSmall = 3072, kFftElemsLarge = 6144;
+// primes from this size up take a Bluestein pass when its work arrays fit
+constexpr int kBluesteinMin = 48;
 
 // Work-groups are dispatched round-robin over the 8 XCDs; neighbouring line
 // blocks share 128-byte lines of the phi-major grid, so consecutive logical
@@ -196,12 +198,64 @@ __device__ void fft_pass_big(double2 *buf, double2 *scr, int n, int F, int Ns, i
   __syncthreads();
 }
 
+// M-point FFTs (M a power of two) over nseq sequences held in LDS.
+template <int E>
+__device__ void fft_pow2(double2 *w, const FftPlan &fp, const FftPass &ps, int nseq,
+                         const double2 *__restrict__ tw, double s) {
+  for (int ip = 0; ip < ps.nm; ip++) {
+    const FftPass q = fp.pass[ps.m0 + ip];
+    switch (q.R) {
+      case 8: fft_pass<E, 8>(w, ps.M, nseq, q.Ns, tw + q.tw_off, s); break;
+      case 4: fft_pass<E, 4>(w, ps.M, nseq, q.Ns, tw + q.tw_off, s); break;
+      default: fft_pass<E, 2>(w, ps.M, nseq, q.Ns, tw + q.tw_off, s); break;
+    }
+  }
+}
+
+// First pass of a large prime radix R as Bluestein's chirp convolution:
+// with c_k = exp(s i pi k^2 / R) (2 r q = r^2 + q^2 - (q-r)^2),
+//   y_q = sum_r x_r w^{r q} = c_q sum_r (x_r c_r) conj(c_{q-r}),
+// a cyclic convolution of length M >= 2R - 1 done by M-point FFTs; the
+// filter's transform (1/M included) is precomputed.  The F (n/R) sequences
+// of the pass are transformed together in wrk (F (n/R) M elements past buf).
+template <int E>
+__device__ void fft_pass_bluestein(double2 *buf, double2 *wrk, const FftPlan &fp,
+                                   const FftPass &ps, int F, const double2 *__restrict__ tw,
+                                   double s) {
+  const int n = fp.n, R = ps.R, M = ps.M, nR = n / R, nseq = F * nR;
+  const int si = s > 0 ? 0 : 1;
+  const double2 *c = tw + ps.chirp_off + si * R, *hat = tw + ps.hat_off + si * M;
+  for (int o = threadIdx.x; o < nseq * M; o += kFftThreads) {
+    const int sq = o / M, m = o - sq * M;
+    const int f = sq / nR, j = sq - f * nR;
+    wrk[o] = m < R ? cmul(buf[(long)f * n + j + m * nR], c[m]) : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  fft_pow2<E>(wrk, fp, ps, nseq, tw, -1.0);
+  for (int o = threadIdx.x; o < nseq * M; o += kFftThreads) wrk[o] = cmul(wrk[o], hat[o % M]);
+  __syncthreads();
+  fft_pow2<E>(wrk, fp, ps, nseq, tw, 1.0);
+  // output q of butterfly j goes to Stockham position j R + q (Ns = 1)
+  for (int o = threadIdx.x; o < nseq * R; o += kFftThreads) {
+    const int sq = o / R, q = o - sq * R;
+    const int f = sq / nR, j = sq - f * nR;
+    buf[(long)f * n + j * R + q] = cmul(wrk[(long)sq * M + q], c[q]);
+  }
+  __syncthreads();
+}
+
 template <int E, bool GEN>
 __device__ void fft_run(double2 *buf, const FftPlan &fp, int F, const double2 *__restrict__ tw,
                         double s) {
   for (int ip = 0; ip < fp.npass; ip++) {
     const FftPass ps = fp.pass[ip];
     const double2 *t = tw + ps.tw_off;
+    if constexpr (GEN) {
+      if (ps.M) {
+        fft_pass_bluestein<E>(buf, buf + (long)F * fp.n, fp, ps, F, tw, s);
+        continue;
+      }
+    }
     switch (ps.R) {
       case 8: fft_pass<E, 8>(buf, fp.n, F, ps.Ns, t, s); break;
       case 4: fft_pass<E, 4>(buf, fp.n, F, ps.Ns, t, s); break;
@@ -826,13 +880,29 @@ int fft_plan_init(sht_plan *p) {
       fp.big = 1;
     }
   }
+  std::sort(big.begin(), big.end(), [](int a, int b) { return a > b; });  // largest first
   fac.insert(fac.begin(), big.begin(), big.end());
-  if ((int)fac.size() > kMaxPass) {
+  // the largest prime as a Bluestein convolution when it is large enough to
+  // pay (R >= kBluesteinMin) and the M-point work arrays fit in LDS
+  int blM = 0;
+  if (!big.empty() && big[0] >= kBluesteinMin) {
+    int M = 1;
+    while (M < 2 * big[0] - 1) M *= 2;
+    if (n + (n / big[0]) * M <= kFftElemsLarge) blM = M;
+  }
+  std::vector<int> mfac;
+  for (int m = blM; m > 1;) {
+    const int r = m % 8 == 0 ? 8 : m % 4 == 0 ? 4 : 2;
+    mfac.push_back(r);
+    m /= r;
+  }
+  if ((int)(fac.size() + mfac.size()) > kMaxPass) {
     set_error("FFT length %d has more than %d factors (unsupported)", n, kMaxPass);
     return SHT_ERR_UNSUPPORTED;
   }
-  const int E = n <= kFftElemsSmall ? kFftElemsSmall : kFftElemsLarge;
-  int F = E / (fp.big ? 2 * n : n);  // complex FFTs per work-group (plus the scratch copy)
+  fp.line_elems = blM ? n + (n / big[0]) * blM : fp.big ? 2 * n : n;
+  const int E = (blM || n > kFftElemsSmall) ? kFftElemsLarge : kFftElemsSmall;
+  int F = E / fp.line_elems;  // complex FFTs per work-group (with their scratch)
   if (F < 1) {
     set_error("FFT length %d exceeds the LDS capacity (%d%s)", n, kFftElemsLarge,
               fp.big ? " with a scratch copy for its prime factors above 13" : "");
@@ -844,6 +914,7 @@ int fft_plan_init(sht_plan *p) {
   while (f2 * 2 <= F) f2 *= 2;
   fp.lines_per_wg = 2 * f2;
   std::vector<double2> tw;
+  const long double kPi = 3.14159265358979323846264338327950288L;
   int Ns = 1;
   for (int R : fac) {
     FftPass ps;
@@ -851,6 +922,49 @@ int fft_plan_init(sht_plan *p) {
     ps.Ns = Ns;
     ps.tw_off = (int)tw.size();
     ps.pad = 0;
+    ps.M = ps.chirp_off = ps.hat_off = ps.m0 = ps.nm = 0;
+    if (Ns == 1 && blM && R == big[0]) {
+      // Bluestein: chirps c_k = exp(s i pi k^2 / R) and the transform of the
+      // filter B (B_m = conj c_m, B_{M-k} = conj c_k, zero between), 1/M
+      // included, for s = +1 then -1; the M-point passes follow the main ones
+      ps.M = blM;
+      ps.chirp_off = (int)tw.size();
+      for (int sg = 1; sg >= -1; sg -= 2)
+        for (int k = 0; k < R; k++) {
+          const long double a = sg * kPi * (long double)(((long long)k * k) % (2 * R)) / R;
+          tw.push_back(make_double2((double)cosl(a), (double)sinl(a)));
+        }
+      ps.hat_off = (int)tw.size();
+      for (int sg = 1; sg >= -1; sg -= 2) {
+        std::vector<long double> br(blM, 0.0L), bi(blM, 0.0L);
+        for (int k = 0; k < R; k++) {
+          const long double a = sg * kPi * (long double)(((long long)k * k) % (2 * R)) / R;
+          br[k] = cosl(a);
+          bi[k] = -sinl(a);
+          if (k) {
+            br[blM - k] = cosl(a);
+            bi[blM - k] = -sinl(a);
+          }
+        }
+        std::vector<long double> rc(blM), rs(blM);  // exp(-2 pi i t / M)
+        for (int t = 0; t < blM; t++) {
+          rc[t] = cosl(-2.0L * kPi * t / blM);
+          rs[t] = sinl(-2.0L * kPi * t / blM);
+        }
+        for (int u = 0; u < blM; u++) {
+          long double sr = 0.0L, si = 0.0L;
+          for (int m = 0; m < blM; m++) {
+            const int t = (int)(((long long)u * m) % blM);
+            sr += br[m] * rc[t] - bi[m] * rs[t];
+            si += br[m] * rs[t] + bi[m] * rc[t];
+          }
+          tw.push_back(make_double2((double)(sr / blM), (double)(si / blM)));
+        }
+      }
+      fp.pass[fp.npass++] = ps;
+      Ns *= R;
+      continue;
+    }
     if (R > 13) {
       // direct-sum pass: all Ns R roots of order Ns R
       for (int e = 0; e < Ns * R; e++) {
@@ -868,7 +982,28 @@ int fft_plan_init(sht_plan *p) {
     fp.pass[fp.npass++] = ps;
     Ns *= R;
   }
-  const int lds = (int)(sizeof(double2) * (size_t)f2 * n * (fp.big ? 2 : 1));
+  if (blM) {
+    // the Bluestein convolution's M-point passes, stored past the main ones
+    fp.pass[0].m0 = fp.npass;
+    fp.pass[0].nm = (int)mfac.size();
+    int ms = 1;
+    for (size_t i = 0; i < mfac.size(); i++) {
+      FftPass q;
+      q.R = mfac[i];
+      q.Ns = ms;
+      q.tw_off = (int)tw.size();
+      q.pad = 0;
+      q.M = q.chirp_off = q.hat_off = q.m0 = q.nm = 0;
+      for (int k = 0; k < ms; k++)
+        for (int r = 0; r < q.R; r++) {
+          const long double a = -2.0L * kPi * (long double)(k * r) / (long double)(ms * q.R);
+          tw.push_back(make_double2((double)cosl(a), (double)sinl(a)));
+        }
+      fp.pass[fp.npass + i] = q;
+      ms *= q.R;
+    }
+  }
+  const int lds = (int)(sizeof(double2) * (size_t)f2 * fp.line_elems);
   bool gen = false;
   for (int R : fac) gen |= !(R == 2 || R == 3 || R == 4 || R == 8);
   p->fft_generic = gen;
@@ -952,7 +1087,7 @@ int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, doub
   const long long nline = (long long)p->nr * p->nth;
   const int B = p->fft.lines_per_wg;
   const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);
-  const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.n * (p->fft.big ? 2 : 1);
+  const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.line_elems;
 #define KSHT_FFT_LAUNCH(K, ...)                                                              \
   do {                                                                                     \
     if (p->fft.pad == kFftElemsSmall) {                                                    \
@@ -980,7 +1115,7 @@ int launch_fft_fwd(sht_plan *p, int ncomp, const double *v, double *X, FourierAd
   const long long nline = (long long)p->nr * p->nth;
   const int B = p->fft.lines_per_wg;
   const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);
-  const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.n * (p->fft.big ? 2 : 1);
+  const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.line_elems;
   if (p->fft.fast)
     KSHT_FFT3_LAUNCH(k_fft3_fwd, kFftNCf, p->fft.lds3, p->nphi, p->fft.r0, p->L, nline, v, X, xa,
                      p->t.twn);

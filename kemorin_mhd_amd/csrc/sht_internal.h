@@ -38,10 +38,16 @@ void set_error(const char *fmt, ...);
 // offset of its twiddle block (Ns*R entries) in the twiddle table.
 struct FftPass {
   int R, Ns, tw_off, pad;
+  // Bluestein form of a large prime pass (first pass only, M = 0 otherwise):
+  // convolution length M, offsets in the twiddle table of the chirps
+  // exp(s i pi k^2 / R), k < R, and of the filter transforms (M entries),
+  // each for s = +1 then s = -1, and the M-point passes at pass[m0 .. m0+nm)
+  int M, chirp_off, hat_off, m0, nm;
 };
 struct FftPlan {
   int n, npass, lines_per_wg, pad;  // pad: LDS elements per work-group
   int big;                          // a prime factor above 13 (LDS scratch copy)
+  int line_elems;                   // LDS elements per complex line (with scratch)
   FftPass pass[kMaxPass];
   // split form n = r0 * np (np a power of two above L)
   int fast, np, r0, lds3, lds3b;  // LDS bytes: forward, backward (whole-block staging)

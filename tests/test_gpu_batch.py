@@ -16,8 +16,9 @@ from kemorin_mhd_amd import (SHT_PREC_FP64, SHT_PREC_SPLIT_INT8, ShellBatch, She
 
 pytestmark = pytest.mark.gpu
 
-# (L, Nr); L = 45 gives Nphi = 138 = 2 * 3 * 23 (a direct-sum FFT pass)
-SHELLS = [(15, 5), (47, 4), (31, 7), (23, 6), (63, 9), (39, 3), (45, 2)]
+# (L, Nr); L = 45 gives Nphi = 138 = 2 * 3 * 23 (a direct-sum FFT pass),
+# L = 105 gives Nphi = 318 = 2 * 3 * 53 (a Bluestein pass)
+SHELLS = [(15, 5), (47, 4), (31, 7), (23, 6), (63, 9), (39, 3), (45, 2), (105, 2)]
 
 
 def _say(*a):

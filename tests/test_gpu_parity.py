@@ -20,6 +20,8 @@ CASES = {
     # Nphi = 3 (L+1) with prime factors above 13 (direct-sum FFT passes)
     "L37_nr3_nth57_nphi114": ShellConfig(37, 3, 57, 114),
     "L45_nr2_nth69_nphi138": ShellConfig(45, 2, 69, 138),
+    # prime 53: a Bluestein pass
+    "L52_nr2_nth80_nphi318": ShellConfig(52, 2, 80, 318),
 }
 TOL = 1e-12
 
