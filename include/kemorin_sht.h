@@ -137,7 +137,8 @@ int sht_exchange_gather_rev(sht_plan *plan, int ncomp, long long n_dst,
  * launch.  No counterpart in the reference, which transforms one shell per
  * call (SS/t_sph_transforms.f90:74,250); it is for callers that carry many
  * uneven shells and would otherwise pay one launch sequence per shell with
- * most of the device idle.  Shells are dealt to parallel graph branches.
+ * most of the device idle.  Shells are dealt to parallel graph branches;
+ * calls that share a plan run in order on one branch.
  * A field set of (0, 0) skips that direction.  The plans' stage timers must
  * be off; the graph keeps the plans' workspaces and the buffer pointers, so
  * neither may be freed, and the plans may not run other work concurrently

@@ -6,8 +6,9 @@
 // A batch captures, once, the backward and forward transforms of every shell
 // (each shell its own plan and buffers) into one graph: the shells are dealt
 // to kBranches parallel branches forked from and joined to one capture
-// stream, so independent shells may run side by side, and the whole set
-// starts with a single hipGraphLaunch.  Replays reuse the captured buffer
+// stream (calls that share a plan stay on one branch, in order), so
+// independent shells may run side by side, and the whole set starts with a
+// single hipGraphLaunch.  Replays reuse the captured buffer
 // pointers; the caller refills the inputs between launches.
 #include <vector>
 
@@ -104,9 +105,19 @@ extern "C" int sht_batch_create(const sht_shell_call *calls, int n, sht_batch **
     if (!hip_ok(hipEventRecord(fork, cap), "fork")) break;
     for (int k = 0; k < kBranches && ok; k++) ok = hip_ok(hipStreamWaitEvent(br[k], fork, 0), "fork");
     if (!ok) break;
+    // calls on one plan share its workspace: they go to the branch of the
+    // plan's first call, in order; distinct plans are dealt round-robin
+    std::vector<int> branch(n);
+    int next = 0;
+    for (int i = 0; i < n; i++) {
+      branch[i] = -1;
+      for (int k = 0; k < i && branch[i] < 0; k++)
+        if (calls[k].plan == calls[i].plan) branch[i] = branch[k];
+      if (branch[i] < 0) branch[i] = next++ % kBranches;
+    }
     for (int i = 0; i < n && !rc; i++) {
       const sht_shell_call &c = calls[i];
-      c.plan->stream = br[i % kBranches];
+      c.plan->stream = br[branch[i]];
       if (3 * c.nvector_b + c.nscalar_b)
         rc = sht_backward(c.plan, c.nvector_b, c.nscalar_b, c.d_WS_rj, c.d_v_rtp);
       if (!rc && 3 * c.nvector_f + c.nscalar_f)

@@ -73,6 +73,38 @@ def test_batch_graph_matches_eager(torch_cuda, precision):
         p.close()
 
 
+def test_batch_calls_sharing_a_plan_run_in_order(torch_cuda):
+    # two calls on one plan (one workspace): the second must see the first
+    # done, so the graph result equals two eager calls in a row
+    torch = torch_cuda
+    c = _shells()[1]
+    p = SphTransPlan(c, max_ncomp=4)
+    ws = [torch.tensor(synthetic_spectra(c, 1, 1, seed=300 + i), device="cuda") for i in range(2)]
+    v = [torch.zeros(c.n_rtp(4), dtype=torch.float64, device="cuda") for _ in range(2)]
+    for i in range(2):
+        p.sph_backward_transforms(1, 1, ws[i], v[i])
+    torch.cuda.synchronize()
+    ref = [x.clone() for x in v]
+    calls = [(p, (1, 1), ws[i], v[i], (0, 0), None, None) for i in range(2)]
+    other = SphTransPlan(_shells()[0], max_ncomp=4)
+    c0 = _shells()[0]
+    wo = torch.tensor(synthetic_spectra(c0, 1, 1, seed=9), device="cuda")
+    vo = torch.zeros(c0.n_rtp(4), dtype=torch.float64, device="cuda")
+    calls.insert(1, (other, (1, 1), wo, vo, (0, 0), None, None))
+    b = ShellBatch(calls)
+    for rep in range(2):
+        for x in v:
+            x.fill_(np.nan)
+        torch.cuda.synchronize()
+        b.launch()
+        b.synchronize()
+        for i in range(2):
+            assert torch.equal(v[i], ref[i]), f"replay {rep} call {i}"
+    b.close()
+    other.close()
+    p.close()
+
+
 def test_batch_rejects_timed_plans(torch_cuda):
     torch = torch_cuda
     c = _shells()[0]
