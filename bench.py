@@ -293,15 +293,21 @@ def kernel_roofline(cfg_d, res, precision, slices):
     nline = cfg_d["nr"] * cfg_d["nth"]
     nslot = 2 * cfg_d["L"] + 1
     tri = slices * (slices + 1) // 2
-    npow = min(cfg_d["nphi"] & -cfg_d["nphi"], 512)  # Nphi = R0 * NP, NP a power of two
+    # the plan's FFT form: split form Nphi = R0 NP (NP the smallest power of two
+    # above L, 32..512) with 4 (backward) and 8 (forward) complex lines per
+    # work-group, else the mixed-radix kernels
+    npow = 32
+    while npow < cfg_d["L"] + 1:
+        npow *= 2
+    fast = npow <= 512 and cfg_d["nphi"] % npow == 0
     split = precision == "split"
     kname = {
         "legendre_backward_vector_kernel": (f"ksht::k_leg_bwd_split<{slices}, 3, true>" if split
                                             else "ksht::k_leg_bwd<true>"),
         "legendre_forward_vector_kernel": (f"ksht::k_leg_fwd_split<{slices}, 3, true>" if split
                                            else "ksht::k_leg_fwd<true>"),
-        "fft_backward": f"ksht::k_fft3_bwd<{npow}>",
-        "fft_forward": f"ksht::k_fft3_fwd<{npow}>",
+        "fft_backward": f"ksht::k_fft3_bwd<{npow}, 4>" if fast else "ksht::k_fft_bwd (mixed radix)",
+        "fft_forward": f"ksht::k_fft3_fwd<{npow}, 8>" if fast else "ksht::k_fft_fwd (mixed radix)",
     }
     stages = {
         "legendre_backward_vector_kernel": ("mfma", 10 * F1),
