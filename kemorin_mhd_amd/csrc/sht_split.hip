@@ -405,8 +405,9 @@ __global__ void __launch_bounds__(kSliceThreads) k_fslice(
         const int t = c * 64 + tt, k = kt * 16 + kk;
         // unconditional loads (clamped into range, zeroed after) so that all
         // of them stay in flight together
+        // (radii past the last read the last one: the line the tile reads anyway)
         const bool ok = f < ng && k < nr && t < nth_h;
-        const int fc = ok ? f : 0, kc = ok ? k : 0, tc = ok ? t : 0;
+        const int fc = f < ng ? f : 0, kc = k < nr ? k : nr - 1, tc = t < nth_h ? t : 0;
         const long long b = base + (long long)(cg + fc) * ga.s_nd + (long long)kc * ga.s_k;
         vn[u] = in[b + (long long)tc * ga.s_t];
         vs[u] = in[b + (long long)(nth - 1 - tc) * ga.s_t];
