@@ -396,8 +396,8 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_fwd(
 }
 
 // ---------------------------------------------------------------------------
-// Split form for Nphi = R0 * NP with NP a power of two, L < NP (the usual
-// Nphi = 3 * 2^k grids).  With j = R0 j2 + j1:
+// Split form for Nphi = R0 * NP with NP a power of two, L < NP <= 1024 (the
+// usual Nphi = 3 * 2^k grids).  With j = R0 j2 + j1:
 //   backward  x_j = Re/Im of sum_m Z_m e^{+2 pi i m j/N}
 //                 = sum_{m'<NP} D^{(j1)}_m' e^{+2 pi i m' j2/NP},
 //             D^{(j1)}_m' = e^{2 pi i m' j1/N} Z_m' + e^{2 pi i (m'+N-NP) j1/N} Z_{m'+N-NP}
@@ -419,8 +419,15 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_fwd(
 // overlaps instead the copy of the next phi decimation with the passes of
 // the current one inside the work-group; its alias twiddles come from a
 // full N-entry table (composing them from the NP roots, as the backward
-// does, would not fit its register budget).
-constexpr int kFftNCb = 4, kFftNCf = 8;
+// does, would not fit its register budget).  NP = 1024 (L up to 1023): a
+// complex line spans two waves (T = 128), the passes synchronise the work-
+// group, and the forward runs NC = 4 without the in-kernel copy of the next
+// decimation (its stage would not fit beside the N-entry table).
+constexpr int kFftNCb = 4;
+template <int NP>
+constexpr int fft_ncf() { return NP <= 512 ? 8 : 4; }
+template <int NP>
+constexpr bool fft_fwd_dma() { return NP <= 512; }
 
 __device__ __forceinline__ int fpad(int i) { return i + (i >> 3); }
 
@@ -444,19 +451,32 @@ constexpr int fft3_tw_off_bwd() {
 }
 template <int NP, int NC>
 constexpr int fft3_tw_off_fwd() {
-  constexpr int a = NC * (NP + NP / 8) + NC * NP, b = (2 * NC + 1) * (NP < 256 ? NP : 256);
+  // with the copy stage past the pass areas, or (no in-kernel copy) the
+  // global-load stage of NP rows x (NC + 1) slots over them
+  constexpr int c = NC * (NP + NP / 8), d = (NC + 1) * NP;
+  constexpr int a = fft_fwd_dma<NP>() ? c + NC * NP : (c > d ? c : d);
+  constexpr int b = (2 * NC + 1) * (NP < 256 ? NP : 256);
   return a > b ? a : b;
 }
 
-// The pass area of a complex line belongs to the wave that transforms it
-// (T = NP/8 <= 64 threads per line, lines never straddle a wave), and one
-// wave's LDS operations complete in issue order: between the passes only
-// the compiler has to keep the stores before the loads and the loads before
-// the next stores.  Callers put a work-group barrier between the passes and
-// any use of the area by other waves.
+// For NP <= 512 the pass area of a complex line belongs to the wave that
+// transforms it (T = NP/8 <= 64 threads per line, lines never straddle a
+// wave), and one wave's LDS operations complete in issue order: between the
+// passes only the compiler has to keep the stores before the loads and the
+// loads before the next stores.  Callers put a work-group barrier between
+// the passes and any use of the area by other waves.
 __device__ __forceinline__ void wave_lds_order() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
+}
+// ordering between passes: wave-local when a complex line fits one wave,
+// else a work-group barrier (every thread runs the passes)
+template <int NP>
+__device__ __forceinline__ void pass_lds_order() {
+  if constexpr (NP / 8 <= 64)
+    wave_lds_order();
+  else
+    __syncthreads();
 }
 
 // tw: roots of unity in LDS, of order NP at stride ts (the NP-entry table
@@ -488,13 +508,13 @@ __device__ __forceinline__ void f3_passes(double2 (&v)[8], double2 *ln, int j,
 #pragma unroll
       for (int r = 0; r < R; r++) ln[fpad((b / Ns) * Ns * R + b % Ns + Ns * r)] = v[i * R + r];
     }
-    wave_lds_order();
+    pass_lds_order<NP>();
     constexpr int R2 = (NP / (Ns * R) >= 8) ? 8 : NP / (Ns * R);
 #pragma unroll
     for (int i = 0; i < 8 / R2; i++)
 #pragma unroll
       for (int r = 0; r < R2; r++) v[i * R2 + r] = ln[fpad(j + T * i + (NP / R2) * r)];
-    wave_lds_order();
+    pass_lds_order<NP>();
     f3_passes<NP, Ns * R>(v, ln, j, tw, ts, s);
   }
 }
@@ -724,7 +744,8 @@ __global__ void __launch_bounds__(NC * NP / 8) k_fft3_fwd(int N, int R0, int L,
   // copied global -> LDS while the FFT of j1 runs, into a stage past the
   // pass areas; row r's slot u holds lines 2 (u ^ (r & (NC - 1))) and +1
   // (XOR swizzle: consecutive threads of a complex line read distinct slots).
-  const bool dma = vec2 && (((unsigned long long)v & 15) == 0) && nnod * 8 < (1ll << 32);
+  const bool dma = fft_fwd_dma<NP>() && vec2 && (((unsigned long long)v & 15) == 0) &&
+                   nnod * 8 < (1ll << 32);
   if (dma) {
     double2 *st2 = lds + NC * (NP + NP / 8);
     const unsigned st2_byte =
@@ -809,14 +830,25 @@ __global__ void __launch_bounds__(NC * NP / 8) k_fft3_fwd(int N, int R0, int L,
   }
   // Z_{N-m} for m = j + T q sits at index NP - m of the lane (T - j) % T of
   // this complex line, in register 7 - q (or (8 - q) % 8 when j = 0)
-  const int lane = threadIdx.x & 63;
-  const int partner = lane - j + (T - j) % T;
   double2 zn[8];
+  if constexpr (T <= 64) {
+    const int lane = threadIdx.x & 63;
+    const int partner = lane - j + (T - j) % T;
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const double2 send = (j == 0) ? zhi[(8 - q) % 8] : zhi[7 - q];
-    zn[q].x = __shfl(send.x, partner, 64);
-    zn[q].y = __shfl(send.y, partner, 64);
+    for (int q = 0; q < 8; q++) {
+      const double2 send = (j == 0) ? zhi[(8 - q) % 8] : zhi[7 - q];
+      zn[q].x = __shfl(send.x, partner, 64);
+      zn[q].y = __shfl(send.y, partner, 64);
+    }
+  } else {
+    // the line spans waves: through LDS (index k = j + T q of the line's
+    // pass area holds zhi[q])
+    __syncthreads();  // every wave done with the pass areas
+#pragma unroll
+    for (int q = 0; q < 8; q++) ln[j + T * q] = zhi[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; q++) zn[q] = ln[(NP - (j + T * q)) & (NP - 1)];
   }
   const double sn = 1.0 / (double)N, tsn = 2.0 / (double)N;
   double *stage = reinterpret_cast<double *>(lds);
@@ -1017,22 +1049,30 @@ int fft_plan_init(sht_plan *p) {
   KSHT_HIP(hipMalloc(&p->t.tw, sizeof(double2) * tw.size()));
   KSHT_HIP(hipMemcpy(p->t.tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice));
   p->device_bytes += (long long)(sizeof(double2) * tw.size());
-  // split form n = r0 * np: np the smallest power of two above L, 32..512
+  // split form n = r0 * np: np the smallest power of two above L, 32..1024
   fp.fast = 0;
   int np = 32;
   while (np < p->L + 1) np *= 2;
-  if (np <= 512 && n % np == 0) {
+  if (np <= 1024 && n % np == 0) {
     fp.fast = 1;
     fp.np = np;
     fp.r0 = n / np;
     int tw_b = 0, tw_f = 0;
+#define KSHT_TW_OFF(NP)                                                                      \
+  case NP:                                                                                   \
+    tw_b = fft3_tw_off_bwd<NP, kFftNCb>();                                                   \
+    tw_f = fft3_tw_off_fwd<NP, fft_ncf<NP>()>();                                             \
+    break;
     switch (np) {
-      case 32: tw_b = fft3_tw_off_bwd<32, kFftNCb>(); tw_f = fft3_tw_off_fwd<32, kFftNCf>(); break;
-      case 64: tw_b = fft3_tw_off_bwd<64, kFftNCb>(); tw_f = fft3_tw_off_fwd<64, kFftNCf>(); break;
-      case 128: tw_b = fft3_tw_off_bwd<128, kFftNCb>(); tw_f = fft3_tw_off_fwd<128, kFftNCf>(); break;
-      case 256: tw_b = fft3_tw_off_bwd<256, kFftNCb>(); tw_f = fft3_tw_off_fwd<256, kFftNCf>(); break;
-      default: tw_b = fft3_tw_off_bwd<512, kFftNCb>(); tw_f = fft3_tw_off_fwd<512, kFftNCf>(); break;
+      KSHT_TW_OFF(32)
+      KSHT_TW_OFF(64)
+      KSHT_TW_OFF(128)
+      KSHT_TW_OFF(256)
+      KSHT_TW_OFF(512)
+      KSHT_TW_OFF(1024)
+      default: break;
     }
+#undef KSHT_TW_OFF
     fp.lds3b = (int)sizeof(double2) * (tw_b + np + fp.r0);
     fp.lds3 = (int)sizeof(double2) * (tw_f + n);
     fp.r0m = fp.r0 == 1 ? 0u : (unsigned)(((1ull << 32) + (unsigned)fp.r0 - 1) / (unsigned)fp.r0);
@@ -1055,7 +1095,7 @@ int fft_plan_init(sht_plan *p) {
   case NP:                                                                                   \
     KSHT_HIP(hipFuncSetAttribute((const void *)k_fft3_bwd<NP, kFftNCb>,                      \
                                  hipFuncAttributeMaxDynamicSharedMemorySize, fp.lds3b));     \
-    KSHT_HIP(hipFuncSetAttribute((const void *)k_fft3_fwd<NP, kFftNCf>,                      \
+    KSHT_HIP(hipFuncSetAttribute((const void *)k_fft3_fwd<NP, fft_ncf<NP>()>,                \
                                  hipFuncAttributeMaxDynamicSharedMemorySize, fp.lds3));      \
     break;
     switch (np) {
@@ -1064,6 +1104,7 @@ int fft_plan_init(sht_plan *p) {
       KSHT_FFT3_ATTR(128)
       KSHT_FFT3_ATTR(256)
       KSHT_FFT3_ATTR(512)
+      KSHT_FFT3_ATTR(1024)
       default: break;
     }
 #undef KSHT_FFT3_ATTR
@@ -1071,17 +1112,12 @@ int fft_plan_init(sht_plan *p) {
   return SHT_OK;
 }
 
-#define KSHT_FFT3_LAUNCH(K, NC, LDS, ...)                                                    \
-  do {                                                                                     \
-    const dim3 g3((unsigned)((nline + 2 * (NC) - 1) / (2 * (NC))), (unsigned)ncomp);       \
-    switch (p->fft.np) {                                                                   \
-      case 32: hipLaunchKernelGGL((K<32, NC>), g3, dim3(4 * (NC)), (LDS), p->stream, __VA_ARGS__); break; \
-      case 64: hipLaunchKernelGGL((K<64, NC>), g3, dim3(8 * (NC)), (LDS), p->stream, __VA_ARGS__); break; \
-      case 128: hipLaunchKernelGGL((K<128, NC>), g3, dim3(16 * (NC)), (LDS), p->stream, __VA_ARGS__); break; \
-      case 256: hipLaunchKernelGGL((K<256, NC>), g3, dim3(32 * (NC)), (LDS), p->stream, __VA_ARGS__); break; \
-      default: hipLaunchKernelGGL((K<512, NC>), g3, dim3(64 * (NC)), (LDS), p->stream, __VA_ARGS__); break; \
-    }                                                                                      \
-  } while (0)
+#define KSHT_FFT3_CASE(K, NP, NCV, LDS, ...)                                                 \
+  case NP: {                                                                               \
+    constexpr int nc_ = (NCV);                                                             \
+    const dim3 g3((unsigned)((nline + 2 * nc_ - 1) / (2 * nc_)), (unsigned)ncomp);         \
+    hipLaunchKernelGGL((K<NP, nc_>), g3, dim3(NP / 8 * nc_), (LDS), p->stream, __VA_ARGS__); \
+  } break;
 
 int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, double *v) {
   const long long nline = (long long)p->nr * p->nth;
@@ -1102,10 +1138,20 @@ int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, doub
         hipLaunchKernelGGL((K<kFftElemsLarge, false>), grid, dim3(kFftThreads), lds, p->stream, __VA_ARGS__); \
     }                                                                                      \
   } while (0)
-  if (p->fft.fast)
-    KSHT_FFT3_LAUNCH(k_fft3_bwd, kFftNCb, p->fft.lds3b, p->nphi, p->fft.r0, p->fft.r0m, p->L, nline, X,
-                     xa, v, p->t.twn);
-  else
+  if (p->fft.fast) {
+#define KSHT_B(NP) KSHT_FFT3_CASE(k_fft3_bwd, NP, kFftNCb, p->fft.lds3b, p->nphi, p->fft.r0,   \
+                                  p->fft.r0m, p->L, nline, X, xa, v, p->t.twn)
+    switch (p->fft.np) {
+      KSHT_B(32)
+      KSHT_B(64)
+      KSHT_B(128)
+      KSHT_B(256)
+      KSHT_B(512)
+      KSHT_B(1024)
+      default: break;
+    }
+#undef KSHT_B
+  } else
     KSHT_FFT_LAUNCH(k_fft_bwd, p->fft, p->L, nline, X, xa, v, p->t.tw);
   KSHT_HIP(hipGetLastError());
   return SHT_OK;
@@ -1116,10 +1162,20 @@ int launch_fft_fwd(sht_plan *p, int ncomp, const double *v, double *X, FourierAd
   const int B = p->fft.lines_per_wg;
   const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);
   const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.line_elems;
-  if (p->fft.fast)
-    KSHT_FFT3_LAUNCH(k_fft3_fwd, kFftNCf, p->fft.lds3, p->nphi, p->fft.r0, p->L, nline, v, X, xa,
-                     p->t.twn);
-  else
+  if (p->fft.fast) {
+#define KSHT_F(NP) KSHT_FFT3_CASE(k_fft3_fwd, NP, fft_ncf<NP>(), p->fft.lds3, p->nphi, p->fft.r0, \
+                                  p->L, nline, v, X, xa, p->t.twn)
+    switch (p->fft.np) {
+      KSHT_F(32)
+      KSHT_F(64)
+      KSHT_F(128)
+      KSHT_F(256)
+      KSHT_F(512)
+      KSHT_F(1024)
+      default: break;
+    }
+#undef KSHT_F
+  } else
     KSHT_FFT_LAUNCH(k_fft_fwd, p->fft, p->L, nline, v, X, xa, p->t.tw);
   KSHT_HIP(hipGetLastError());
   return SHT_OK;
