@@ -208,8 +208,9 @@ def test_cfg2_fp64_linearity(torch_cuda):
 def test_truncation_L1023(torch_cuda, nth, nphi):
     """Config 4's truncation L=1023, on the smallest grid it allows (Ntheta =
     L+1 = 1024, Nphi = 2L+2 = 2048) and on config 4's own grid (Ntheta =
-    1536, Nphi = 3072); both are past the split-form FFT, so the mixed-radix
-    FFT runs.  2 radii keep the oracle short (the radii are independent):
+    1536, Nphi = 3072); both run the split-form FFT with N' = 1024 (a complex
+    line across two waves).  2 radii keep the oracle short (the radii are
+    independent):
     FP64 engine against the oracle (1e-12 per component), split engine
     against the FP64 engine (1e-11), whole backward (2 vector + 2 scalar) and
     forward (3 vector)."""
