@@ -101,6 +101,24 @@ int sht_backward_host(sht_plan *plan, int nvector, int nscalar,
 int sht_forward_host(sht_plan *plan, int nvector, int nscalar,
                      const double *v_rtp, double *WR_rj);
 
+/* Backward transform at the poles and the centre, device buffers.
+ * Replaces pole_b_transform (SS/t_sph_transforms.f90:203-245), i.e.
+ * pole_backward_transforms (SS/pole_sph_transform.f90:107-162) on one rank.
+ * d_WS_rj as for sht_backward; d_v_pole[i + (2*Nr + 1)*nd] receives rows
+ * i = 0..Nr-1 (north pole), Nr..2Nr-1 (south pole) and, for vector
+ * components when with_center != 0 (iflag_MESH_w_center), 2Nr (the centre,
+ * from the innermost radius).  Vectors come out Cartesian: (x, y, z) =
+ * (v_theta, v_phi, v_r) at phi = 0 in the north, (-v_theta, v_phi, -v_r)
+ * in the south.  Centre entries not written keep their values, as in the
+ * reference. */
+int sht_pole_backward(sht_plan *plan, int nvector, int nscalar,
+                      const double *d_WS_rj, double *d_v_pole, int with_center);
+/* sht_pole_backward, then sht_backward on the same spectra.
+ * Replaces sph_b_trans_w_poles (SS/t_sph_transforms.f90:127-199). */
+int sht_backward_with_poles(sht_plan *plan, int nvector, int nscalar,
+                            const double *d_WS_rj, double *d_v_rtp,
+                            double *d_v_pole, int with_center);
+
 /* Stage entry points on the reference's buffers, for callers that run work
  * between stages (the Coriolis hooks of MS/sph_trans_w_coriols.f90:146,233).
  * Legendre: replaces sel_backward_legendre_trans / sel_forward_legendre_trans

@@ -110,6 +110,7 @@ struct Tables {
   double2 *tw = nullptr;                        // FFT twiddles
   double2 *twn = nullptr;                       // exp(-2 pi i k / Nphi), k < Nphi
   long long *slot_off_X = nullptr, *slot_off_rtm = nullptr;  // 2L+1 each
+  double *pole = nullptr;  // P_l^0, dP_l^1/dtheta at theta = 0 and pi, 4 (L+1)
 };
 
 }  // namespace ksht
@@ -179,6 +180,7 @@ int launch_gather_rev(sht_plan *p, int ncomp, long long n_dst, long long n_src,
                       const int *irev, const double *src, double *dst);
 // P table generation (sht_plan.hip)
 int generate_p_tables(sht_plan *p);
+int pole_tables_init(sht_plan *p);
 // split precision (sht_split.hip)
 int split_init(sht_plan *p);
 void split_free(sht_plan *p);

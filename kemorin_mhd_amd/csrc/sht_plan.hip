@@ -226,7 +226,7 @@ static void free_plan(sht_plan *p) {
   void *ptrs[] = {p->t.offb, p->t.offf, p->t.ipad, p->t.jst, p->t.Pb, p->t.dPb,
                   p->t.Pf, p->t.dPf, p->t.radius, p->t.a_r, p->t.weight, p->t.asin_t,
                   p->t.cos_t, p->t.sin_t, p->t.tw, p->t.twn, p->t.slot_off_X, p->t.slot_off_rtm,
-                  p->X};
+                  p->t.pole, p->X};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   split_free(p);
@@ -333,6 +333,7 @@ extern "C" int sht_plan_create(const sht_desc *d, sht_plan **out) {
   CHECK(upload(&p->t.slot_off_X, so_X.data(), p->nslot, &p->device_bytes));
   CHECK(upload(&p->t.slot_off_rtm, so_rtm.data(), p->nslot, &p->device_bytes));
   CHECK(generate_p_tables(p));
+  CHECK(pole_tables_init(p));
   CHECK(fft_plan_init(p));
   p->x_elems = (long long)p->max_ncomp * p->nslot * nline;
   if (hipMalloc(&p->X, sizeof(double) * (size_t)p->x_elems) != hipSuccess) {

@@ -121,6 +121,30 @@
           real(c_double), intent(out) :: WR_rj(*)
         end function sht_forward_host
 !
+!>      pole_b_transform (SS/t_sph_transforms.f90:203-245), device
+!!      buffers; v_pole(2*Nr+1, ncomp), with_center = 1 for
+!!      iflag_MESH_w_center
+        integer(c_int) function sht_pole_backward(plan, nvector,       &
+     &                          nscalar, d_WS_rj, d_v_pole,            &
+     &                          with_center)                           &
+     &                          bind(C, name = 'sht_pole_backward')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: plan
+          integer(c_int), value :: nvector, nscalar, with_center
+          type(c_ptr), value :: d_WS_rj, d_v_pole
+        end function sht_pole_backward
+!
+!>      sph_b_trans_w_poles (SS/t_sph_transforms.f90:127-199), device
+        integer(c_int) function sht_backward_with_poles(plan,          &
+     &                          nvector, nscalar, d_WS_rj, d_v_rtp,    &
+     &                          d_v_pole, with_center)                 &
+     &                          bind(C, name = 'sht_backward_with_poles')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: plan
+          integer(c_int), value :: nvector, nscalar, with_center
+          type(c_ptr), value :: d_WS_rj, d_v_rtp, d_v_pole
+        end function sht_backward_with_poles
+!
 !>      Stage entry points (sel_[backward|forward]_legendre_trans,
 !!      back_FFT_select_from_recv / fwd_FFT_select_to_send,
 !!      calypso_sph_comm_N), device buffers in the reference layouts

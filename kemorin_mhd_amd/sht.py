@@ -71,6 +71,9 @@ def lib() -> ctypes.CDLL:
         for nm in ("sht_fft_backward", "sht_fft_forward", "sht_exchange_rtm_to_rtp",
                    "sht_exchange_rtp_to_rtm"):
             getattr(L, nm).argtypes = [_vp, ctypes.c_int, _vp, _vp]
+        L.sht_pole_backward.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int]
+        L.sht_backward_with_poles.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
+                                              ctypes.c_int]
         L.sht_exchange_gather_rev.argtypes = [_vp, ctypes.c_int, ctypes.c_longlong,
                                               ctypes.c_longlong, _vp, _vp, _vp]
         L.sht_plan_timer_enable.argtypes = [_vp, ctypes.c_int]
@@ -154,6 +157,9 @@ class ShellConfig:
 
     def n_rtm(self, ncomp):
         return ncomp * self.nth * self.nr * self.nslot
+
+    def n_pole(self, ncomp):
+        return ncomp * (2 * self.nr + 1)
 
 
 class SphTransPlan:
@@ -252,6 +258,24 @@ class SphTransPlan:
         self._bind(v_rtp, WR)
         _check(lib().sht_forward(self._h, nvector, nscalar, _ptr(v_rtp), _ptr(WR)),
                "sht_forward")
+
+    def pole_b_transform(self, nvector: int, nscalar: int, WS, v_pole, with_center=False):
+        """spectra -> values at the poles (rows 0..Nr-1 north, Nr..2Nr-1
+        south) and, for vectors with a centre, row 2Nr; v_pole holds
+        ncomp * (2 Nr + 1) doubles, component-major (SS/t_sph_transforms.f90:
+        203-245)."""
+        self._bind(WS, v_pole)
+        _check(lib().sht_pole_backward(self._h, nvector, nscalar, _ptr(WS), _ptr(v_pole),
+                                       int(bool(with_center))), "sht_pole_backward")
+
+    def sph_b_trans_w_poles(self, nvector: int, nscalar: int, WS, v_rtp, v_pole,
+                            with_center=False):
+        """pole_b_transform and sph_backward_transforms on the same spectra
+        (SS/t_sph_transforms.f90:127-199)."""
+        self._bind(WS, v_rtp, v_pole)
+        _check(lib().sht_backward_with_poles(self._h, nvector, nscalar, _ptr(WS), _ptr(v_rtp),
+                                             _ptr(v_pole), int(bool(with_center))),
+               "sht_backward_with_poles")
 
     def backward_host(self, nvector, nscalar, WS: np.ndarray) -> np.ndarray:
         ncomp = 3 * nvector + nscalar

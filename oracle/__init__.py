@@ -57,6 +57,8 @@ def lib() -> ctypes.CDLL:
             getattr(L, nm).argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int]
         for nm in ("orc_rfft_fwd", "orc_rfft_bwd"):
             getattr(L, nm).argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_pole_backward.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                        ctypes.c_int]
         L.orc_num_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -218,6 +220,21 @@ class OraclePlan:
             raise ValueError("org loops need full tables")
         out = np.empty(self.n_rtp(ncomp))
         lib().orc_sht_backward(self._h, nv, ns, _ptr(WS_rj), _ptr(out), 0 if variant == "org" else 1)
+        return out
+
+    def n_pole(self, ncomp):
+        return ncomp * (2 * self.nr + 1)
+
+    def pole_backward(self, nv, ns, WS_rj, center=False, v_pole=None):
+        """Pole (and centre) values; rows the transform leaves alone keep
+        the content of ``v_pole`` (zeros if not given)."""
+        ncomp = 3 * nv + ns
+        WS_rj = np.ascontiguousarray(WS_rj, dtype=np.float64).ravel()
+        assert WS_rj.size == self.n_rj(ncomp)
+        out = (np.zeros(self.n_pole(ncomp)) if v_pole is None
+               else np.array(v_pole, dtype=np.float64).ravel())
+        assert out.size == self.n_pole(ncomp)
+        lib().orc_pole_backward(self._h, nv, ns, _ptr(WS_rj), _ptr(out), 1 if center else 0)
         return out
 
     def forward(self, nv, ns, v_rtp, variant="sym"):

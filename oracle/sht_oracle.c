@@ -1126,3 +1126,92 @@ void orc_sht_forward(const orc_plan *p, int nv, int ns, const double *v_rtp,
   free(rtm);
   free(rtp);
 }
+
+/* ------------------------------------------------------------------ */
+/* Backward transform at the poles and the centre.
+ * Pole tables: SP/schmidt_poly_on_rtm_grid.f90:201-266
+ * (set_lagender_pole_rlm): the Schmidt recurrence at theta = 0 and at
+ * theta = 4 atan(1); only P of order 0 and dP/dtheta of order 1 are read.
+ * Poles: SS/schmidt_b_trans_at_pole.f90:88-141 (scalar) and :145-289
+ * (vector); centre: SS/schmidt_b_trans_at_center.f90:45-62 (scalar, leaves
+ * the row as it is) and :66-115 (vector).  The vector centre follows that
+ * code (x from order -1, y from order +1), not its header comment (which
+ * names the cos mode for x); nothing in the reference pins the centre row
+ * beyond the code, so its parity is unpinned.  Rows of v_pole: 0..nr-1 north
+ * pole, nr..2nr-1 south pole, 2nr the centre (nnod_pole = 2nr + 1,
+ * SP/t_spheric_rtp_data.f90:187); v_pole[i + nnod_pole*nd].  The centre row
+ * is written for vector components only and only when center != 0
+ * (iflag_MESH_w_center); every other centre entry keeps its value. */
+void orc_pole_backward(const orc_plan *p, int nv, int ns, const double *WS,
+                       double *v_pole, int center) {
+  const int L = p->L, nr = p->nr, ncomp = 3 * nv + ns, npole = 2 * nr + 1;
+  const size_t jmax = (size_t)p->jmax;
+  const int j0 = p->lstack[L], jm = p->lstack[L - 1], jp = p->lstack[L + 1];
+  double *pn = (double *)calloc((size_t)(L + 3), sizeof(double));
+  double *ps = (double *)calloc((size_t)(L + 3), sizeof(double));
+  double *dn = (double *)calloc((size_t)(L + 3), sizeof(double));
+  double *ds = (double *)calloc((size_t)(L + 3), sizeof(double));
+  double *tmp = (double *)calloc((size_t)(L + 3), sizeof(double));
+  const double pi = 4.0 * atan(1.0);
+  int k, nd, l;
+  orc_schmidt_legendres_m(L, 0, 0.0, pn, tmp);
+  orc_schmidt_legendres_m(L, 1, 0.0, tmp, dn);
+  orc_schmidt_legendres_m(L, 0, pi, ps, tmp);
+  orc_schmidt_legendres_m(L, 1, pi, tmp, ds);
+  for (nd = 0; nd < ncomp; nd++)
+    for (k = 0; k < 2 * nr; k++) v_pole[k + (size_t)npole * nd] = 0.0;
+#define SP(c, j, kk) WS[(c) + (size_t)ncomp * ((size_t)(j) + jmax * (size_t)(kk))]
+#define VP(i, c) v_pole[(size_t)(i) + (size_t)npole * (size_t)(c)]
+  for (k = 0; k < nr; k++) {
+    const int kn = k, ks = k + nr;
+    const double a = p->a_r[k];
+    for (nd = 0; nd < nv; nd++) {
+      const int cx = 3 * nd, cy = 3 * nd + 1, cz = 3 * nd + 2;
+      for (l = 0; l <= L; l++) { /* order 0: pol (c = 3nd) */
+        const int j = j0 + l;
+        VP(kn, cz) = VP(kn, cz) + SP(3 * nd, j, k) * pn[l] * p->g3[j];
+        VP(ks, cz) = VP(ks, cz) - SP(3 * nd, j, k) * ps[l] * p->g3[j];
+      }
+      for (l = 1; l <= L; l++) { /* order -1 (1s) */
+        const int j = jm + l - 1;
+        VP(kn, cx) = VP(kn, cx) + SP(3 * nd + 2, j, k) * dn[l];
+        VP(kn, cy) = VP(kn, cy) + SP(3 * nd + 1, j, k) * dn[l];
+        VP(ks, cx) = VP(ks, cx) + SP(3 * nd + 2, j, k) * ds[l];
+        VP(ks, cy) = VP(ks, cy) - SP(3 * nd + 1, j, k) * ds[l];
+      }
+      for (l = 1; l <= L; l++) { /* order +1 (1c) */
+        const int j = jp + l - 1;
+        VP(kn, cx) = VP(kn, cx) + SP(3 * nd + 1, j, k) * dn[l];
+        VP(kn, cy) = VP(kn, cy) - SP(3 * nd + 2, j, k) * dn[l];
+        VP(ks, cx) = VP(ks, cx) - SP(3 * nd + 1, j, k) * ds[l];
+        VP(ks, cy) = VP(ks, cy) - SP(3 * nd + 2, j, k) * ds[l];
+      }
+      VP(kn, cx) = VP(kn, cx) * a;
+      VP(kn, cy) = VP(kn, cy) * a;
+      VP(kn, cz) = VP(kn, cz) * a * a;
+      VP(ks, cx) = VP(ks, cx) * a;
+      VP(ks, cy) = VP(ks, cy) * a;
+      VP(ks, cz) = VP(ks, cz) * a * a;
+    }
+    for (nd = 3 * nv; nd < ncomp; nd++)
+      for (l = 0; l <= L; l++) {
+        VP(kn, nd) = VP(kn, nd) + SP(nd, j0 + l, k) * pn[l];
+        VP(ks, nd) = VP(ks, nd) + SP(nd, j0 + l, k) * ps[l];
+      }
+  }
+  if (center) {
+    const double a = p->a_r[0];
+    for (nd = 0; nd < nv; nd++) {
+      VP(2 * nr, 3 * nd + 2) = 2.0 * SP(3 * nd, j0 + 1, 0) * a * a;
+      VP(2 * nr, 3 * nd) = -(2.0 * SP(3 * nd, jm, 0) * a * a);
+      VP(2 * nr, 3 * nd + 1) = -(2.0 * SP(3 * nd, jp, 0) * a * a);
+    }
+  }
+#undef SP
+#undef VP
+  free(pn);
+  free(ps);
+  free(dn);
+  free(ds);
+  free(tmp);
+}

@@ -102,6 +102,20 @@ void orc_fft_bwd_from_rtp(const orc_plan *p, int ncomp, const double *rtp,
 void orc_fft_fwd_to_rtp(const orc_plan *p, int ncomp, const double *v_rtp,
                         double *rtp);
 
+/* Backward transform at the poles (rows 0..nr-1 north, nr..2nr-1 south)
+ * and, when center != 0, the centre (row 2nr) of v_pole[i + (2nr+1)*nd],
+ * from the same rj buffer as orc_sht_backward.  Vector components come out
+ * as (x, y, z) = (v_theta, v_phi, v_r) at phi = 0 in the north and
+ * (-v_theta, v_phi, -v_r) in the south; the centre row of scalars (and of
+ * everything when center == 0) is left as it is.
+ * SS/pole_sph_transform.f90:107-162, SS/schmidt_b_trans_at_pole.f90,
+ * SS/schmidt_b_trans_at_center.f90, SP/schmidt_poly_on_rtm_grid.f90:201-266.
+ * Pinned: the pole tables by closed forms, the pole rows by the limit of the
+ * full-grid backward at phi = 0 and by uniform fields.  The vector centre
+ * row: parity unpinned (restated from the code only). */
+void orc_pole_backward(const orc_plan *p, int nv, int ns, const double *WS_rj,
+                       double *v_pole, int center);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus
