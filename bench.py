@@ -58,7 +58,57 @@ def parse():
     ap.add_argument("--shells", type=int, default=0,
                     help="only run the batched-shell workload: this many seeded uneven shells, "
                          "one HIP graph launch against a loop of per-shell calls")
+    ap.add_argument("--nonlinear", action="store_true",
+                    help="only time the grid-space products of one MHD step (all six terms) "
+                         "on the configuration's grid")
     return ap.parse_args()
+
+
+def run_nonlinear(args, torch):
+    """The grid-space products of one MHD step on the configuration's grid:
+    inputs u, w, B, J (vectors) and T, C (scalars), 14 components; outputs
+    w x u, J x B, u x B, u T, u C and the Coriolis term, 18 components, all in
+    one launch.  Algorithmic traffic: (14 + 18) * 8 bytes per grid point.
+    Timed with HIP events on the plan's stream over the timed launches."""
+    from kemorin_mhd_amd import ShellConfig, SphTransPlan
+    c = CONFIGS[args.config]
+    cfg = ShellConfig(c["L"], c["nr"], c["nth"], c["nphi"])
+    nnod = cfg.nr * cfg.nth * cfg.nphi
+    idx = {"velo": 0, "vort": 3, "magne": 6, "current": 9, "temp": 12, "light": 13,
+           "m_advect": 0, "lorentz": 3, "vp_induct": 6, "h_flux": 9, "c_flux": 12,
+           "coriolis": 15}
+    coef = {"m_advect": 1.0, "lorentz": 0.5, "vp_induct": 1.0, "h_flux": -1.0, "c_flux": -1.0,
+            "coriolis": 2.0}
+    plan = SphTransPlan(cfg, max_ncomp=1)
+    stream = torch.cuda.Stream()
+    plan.set_stream(stream.cuda_stream)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(7)
+    with torch.cuda.stream(stream):
+        fld = torch.randn(14 * nnod, generator=gen, dtype=torch.float64, device="cuda")
+        frc = torch.empty(18 * nnod, dtype=torch.float64, device="cuda")
+    stream.synchronize()
+    for _ in range(args.warmup):
+        plan.nonlinear_terms_in_rtp(idx, coef, 14, fld, 18, frc)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        plan.nonlinear_terms_in_rtp(idx, coef, 14, fld, 18, frc)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    nbytes = 32 * 8 * nnod
+    print(json.dumps({"metric": "grid-space products of one MHD step (6 terms, one launch)",
+                      "value": nbytes / ms / 1e6, "unit": "GB/s", "ms_per_launch": ms,
+                      "steps": args.steps, "warmup": args.warmup, "dtype": "f64",
+                      "data": "synthetic (seeded Gaussian fields)",
+                      "config": {"workload": f"{args.config}: Nr={cfg.nr} Ntheta={cfg.nth} "
+                                             f"Nphi={cfg.nphi}, 14 input and 18 output components"},
+                      "roofline": {"bound": "hbm", "achieved": nbytes / ms / 1e6,
+                                   "peak": 8000.0, "unit": "GB/s",
+                                   "frac": nbytes / ms / 1e6 / 8000.0,
+                                   "algorithmic_bytes": nbytes}}))
+    plan.close()
 
 
 def run_shells(args, torch):
@@ -392,6 +442,9 @@ def main():
         return
     if args.shells > 0:
         run_shells(args, torch)
+        return
+    if args.nonlinear:
+        run_nonlinear(args, torch)
         return
     cfg_d = CONFIGS[args.config]
     from kemorin_mhd_amd import sht as S

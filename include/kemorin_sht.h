@@ -119,6 +119,29 @@ int sht_backward_with_poles(sht_plan *plan, int nvector, int nscalar,
                             const double *d_WS_rj, double *d_v_rtp,
                             double *d_v_pole, int with_center);
 
+/* Grid-space products of one MHD step, on the grid layout of sht_backward's
+ * output and sht_forward's input.  Replaces nonlinear_terms_in_rtp
+ * (MS/cal_nonlinear_sph_MHD.f90:64-128): each output is a vector written at
+ * its first component (0-based) of d_frc_rtp, -1 = term off:
+ *   m_advect  = coef_velo     (vort x velo)
+ *   lorentz   = coef_lor      (current x magne)
+ *   vp_induct = coef_induct   (velo x magne)
+ *   h_flux    = coef_h_advect  velo * temp
+ *   c_flux    = coef_c_advect  velo * light
+ *   coriolis  = -coef_cor (omega x velo), omega = (cos theta, -sin theta, 0)
+ * Inputs are first components in d_fld_rtp (vectors in r, theta, phi); only
+ * those an enabled term reads must be set (-1 otherwise).  Components of
+ * d_frc_rtp no term writes are left alone; d_fld_rtp and d_frc_rtp must not
+ * overlap. */
+typedef struct {
+  int i_velo, i_vort, i_magne, i_current, i_temp, i_light;
+  int i_m_advect, i_lorentz, i_vp_induct, i_h_flux, i_c_flux, i_coriolis;
+  double coef_velo, coef_lor, coef_induct, coef_h_advect, coef_c_advect, coef_cor;
+} sht_nonlinear_desc;
+int sht_nonlinear_rtp(sht_plan *plan, const sht_nonlinear_desc *desc,
+                      int ncomp_fld, const double *d_fld_rtp,
+                      int ncomp_frc, double *d_frc_rtp);
+
 /* Stage entry points on the reference's buffers, for callers that run work
  * between stages (the Coriolis hooks of MS/sph_trans_w_coriols.f90:146,233).
  * Legendre: replaces sel_backward_legendre_trans / sel_forward_legendre_trans

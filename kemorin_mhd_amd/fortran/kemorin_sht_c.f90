@@ -28,6 +28,19 @@
         integer(c_int) :: flags
       end type sht_desc
 !
+!>    Grid-space products (nonlinear_terms_in_rtp): first components,
+!!    0-based, -1 = absent / term off
+      type, bind(C) :: sht_nonlinear_desc
+        integer(c_int) :: i_velo = -1, i_vort = -1, i_magne = -1
+        integer(c_int) :: i_current = -1, i_temp = -1, i_light = -1
+        integer(c_int) :: i_m_advect = -1, i_lorentz = -1
+        integer(c_int) :: i_vp_induct = -1, i_h_flux = -1
+        integer(c_int) :: i_c_flux = -1, i_coriolis = -1
+        real(c_double) :: coef_velo = 0.0d0, coef_lor = 0.0d0
+        real(c_double) :: coef_induct = 0.0d0, coef_h_advect = 0.0d0
+        real(c_double) :: coef_c_advect = 0.0d0, coef_cor = 0.0d0
+      end type sht_nonlinear_desc
+!
       interface
 !
         integer(c_int) function sht_plan_create(desc, plan)           &
@@ -133,6 +146,19 @@
           integer(c_int), value :: nvector, nscalar, with_center
           type(c_ptr), value :: d_WS_rj, d_v_pole
         end function sht_pole_backward
+!
+!>      nonlinear_terms_in_rtp (MS/cal_nonlinear_sph_MHD.f90:64-128),
+!!      device buffers in the grid layout
+        integer(c_int) function sht_nonlinear_rtp(plan, desc,          &
+     &                          ncomp_fld, d_fld_rtp, ncomp_frc,       &
+     &                          d_frc_rtp)                             &
+     &                          bind(C, name = 'sht_nonlinear_rtp')
+          import :: c_int, c_ptr, sht_nonlinear_desc
+          type(c_ptr), value :: plan
+          type(sht_nonlinear_desc), intent(in) :: desc
+          integer(c_int), value :: ncomp_fld, ncomp_frc
+          type(c_ptr), value :: d_fld_rtp, d_frc_rtp
+        end function sht_nonlinear_rtp
 !
 !>      sph_b_trans_w_poles (SS/t_sph_transforms.f90:127-199), device
         integer(c_int) function sht_backward_with_poles(plan,          &

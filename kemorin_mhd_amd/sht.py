@@ -46,6 +46,19 @@ class _Desc(ctypes.Structure):
                 ("n_slices", ctypes.c_int), ("flags", ctypes.c_int)]
 
 
+class _NlDesc(ctypes.Structure):
+    _fields_ = ([(f"i_{n}", ctypes.c_int) for n in
+                 ("velo", "vort", "magne", "current", "temp", "light", "m_advect", "lorentz",
+                  "vp_induct", "h_flux", "c_flux", "coriolis")] +
+                [(f"coef_{n}", ctypes.c_double) for n in
+                 ("velo", "lor", "induct", "h_advect", "c_advect", "cor")])
+
+
+NL_INPUTS = ("velo", "vort", "magne", "current", "temp", "light")
+NL_OUTPUTS = ("m_advect", "lorentz", "vp_induct", "h_flux", "c_flux", "coriolis")
+_NL_COEF = dict(zip(NL_OUTPUTS, ("velo", "lor", "induct", "h_advect", "c_advect", "cor")))
+
+
 class _ShellCall(ctypes.Structure):
     _fields_ = [("plan", _vp), ("nvector_b", ctypes.c_int), ("nscalar_b", ctypes.c_int),
                 ("d_WS_rj", _vp), ("d_v_rtp", _vp), ("nvector_f", ctypes.c_int),
@@ -74,6 +87,8 @@ def lib() -> ctypes.CDLL:
         L.sht_pole_backward.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int]
         L.sht_backward_with_poles.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp,
                                               ctypes.c_int]
+        L.sht_nonlinear_rtp.argtypes = [_vp, ctypes.POINTER(_NlDesc), ctypes.c_int, _vp,
+                                        ctypes.c_int, _vp]
         L.sht_exchange_gather_rev.argtypes = [_vp, ctypes.c_int, ctypes.c_longlong,
                                               ctypes.c_longlong, _vp, _vp, _vp]
         L.sht_plan_timer_enable.argtypes = [_vp, ctypes.c_int]
@@ -276,6 +291,22 @@ class SphTransPlan:
         _check(lib().sht_backward_with_poles(self._h, nvector, nscalar, _ptr(WS), _ptr(v_rtp),
                                              _ptr(v_pole), int(bool(with_center))),
                "sht_backward_with_poles")
+
+    def nonlinear_terms_in_rtp(self, idx: dict, coef: dict, ncomp_fld: int, fld_rtp,
+                               ncomp_frc: int, frc_rtp):
+        """Grid-space products (MS/cal_nonlinear_sph_MHD.f90:64-128) on device
+        buffers in the grid layout.  idx maps input names (velo, vort, magne,
+        current, temp, light) and output names (m_advect, lorentz, vp_induct,
+        h_flux, c_flux, coriolis) to first components; absent outputs are off.
+        coef maps output names to their coefficients."""
+        d = _NlDesc()
+        for n in NL_INPUTS + NL_OUTPUTS:
+            setattr(d, f"i_{n}", int(idx.get(n, -1)))
+        for n in NL_OUTPUTS:
+            setattr(d, f"coef_{_NL_COEF[n]}", float(coef.get(n, 0.0)))
+        self._bind(fld_rtp, frc_rtp)
+        _check(lib().sht_nonlinear_rtp(self._h, ctypes.byref(d), ncomp_fld, _ptr(fld_rtp),
+                                       ncomp_frc, _ptr(frc_rtp)), "sht_nonlinear_rtp")
 
     def backward_host(self, nvector, nscalar, WS: np.ndarray) -> np.ndarray:
         ncomp = 3 * nvector + nscalar

@@ -59,6 +59,8 @@ def lib() -> ctypes.CDLL:
             getattr(L, nm).argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.orc_pole_backward.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp,
                                         ctypes.c_int]
+        L.orc_nonlinear_rtp.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), _dp,
+                                        ctypes.c_int, _dp, ctypes.c_int, _dp]
         L.orc_num_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -235,6 +237,21 @@ class OraclePlan:
                else np.array(v_pole, dtype=np.float64).ravel())
         assert out.size == self.n_pole(ncomp)
         lib().orc_pole_backward(self._h, nv, ns, _ptr(WS_rj), _ptr(out), 1 if center else 0)
+        return out
+
+    NL_IN = ("velo", "vort", "magne", "current", "temp", "light")
+    NL_OUT = ("m_advect", "lorentz", "vp_induct", "h_flux", "c_flux", "coriolis")
+
+    def nonlinear_rtp(self, idx: dict, coef: dict, ncomp_fld, fld, ncomp_frc, frc=None):
+        """Grid-space products; idx maps field names (NL_IN + NL_OUT) to
+        their first component, coef maps output names to coefficients.
+        Components of frc no term writes keep their values."""
+        fld = np.ascontiguousarray(fld, dtype=np.float64).ravel()
+        assert fld.size == self.n_rtp(ncomp_fld)
+        out = np.zeros(self.n_rtp(ncomp_frc)) if frc is None else np.array(frc, dtype=np.float64).ravel()
+        ia = (ctypes.c_int * 12)(*[idx.get(k, -1) for k in self.NL_IN + self.NL_OUT])
+        ca = np.array([coef.get(k, 0.0) for k in self.NL_OUT], dtype=np.float64)
+        lib().orc_nonlinear_rtp(self._h, ia, _ptr(ca), ncomp_fld, _ptr(fld), ncomp_frc, _ptr(out))
         return out
 
     def forward(self, nv, ns, v_rtp, variant="sym"):

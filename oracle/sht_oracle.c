@@ -1215,3 +1215,67 @@ void orc_pole_backward(const orc_plan *p, int nv, int ns, const double *WS,
   free(ds);
   free(tmp);
 }
+
+/* ------------------------------------------------------------------ */
+/* Grid-space products of the MHD step, MS/cal_nonlinear_sph_MHD.f90:64-128
+ * (nonlinear_terms_in_rtp).  Cross products with a coefficient
+ * (CORE_src/BASE/cal_products_smp.f90:153-170), vector times scalar
+ * (:218-233) and the Coriolis term -coef (omega x u), omega = (cos theta,
+ * -sin theta, 0) in (r, theta, phi) (MHD_src/sph_MHD/const_wz_coriolis_rtp.f90:
+ * 117-156).  Fields are field-major on the rtp grid, f[inod + nnod*nd],
+ * inod = k + nr*(l_theta + nth*i_phi).
+ * idx: first component (0-based, -1 = absent / off) of velo, vort, magne,
+ * current, temp, light (inputs) and m_advect, lorentz, vp_induct, h_flux,
+ * c_flux, coriolis (outputs).  coef: velo, lorentz, induct, h_advect,
+ * c_advect, coriolis. */
+static void cross_coef(size_t n, double c, const double *a, const double *b,
+                       double *o) {
+  size_t i;
+  for (i = 0; i < n; i++) {
+    const double a1 = a[i], a2 = a[i + n], a3 = a[i + 2 * n];
+    const double b1 = b[i], b2 = b[i + n], b3 = b[i + 2 * n];
+    o[i] = (a2 * b3 - a3 * b2) * c;
+    o[i + n] = (a3 * b1 - a1 * b3) * c;
+    o[i + 2 * n] = (a1 * b2 - a2 * b1) * c;
+  }
+}
+void orc_nonlinear_rtp(const orc_plan *p, const int idx[12],
+                       const double coef[6], int ncomp_fld, const double *fld,
+                       int ncomp_frc, double *frc) {
+  const size_t n = (size_t)p->nr * p->nth * p->nphi;
+  const double *u = idx[0] >= 0 ? fld + n * idx[0] : NULL;
+  const double *w = idx[1] >= 0 ? fld + n * idx[1] : NULL;
+  const double *b = idx[2] >= 0 ? fld + n * idx[2] : NULL;
+  const double *j = idx[3] >= 0 ? fld + n * idx[3] : NULL;
+  const double *tt = idx[4] >= 0 ? fld + n * idx[4] : NULL;
+  const double *cc = idx[5] >= 0 ? fld + n * idx[5] : NULL;
+  (void)ncomp_fld;
+  (void)ncomp_frc;
+  if (idx[6] >= 0) cross_coef(n, coef[0], w, u, frc + n * idx[6]);
+  if (idx[7] >= 0) cross_coef(n, coef[1], j, b, frc + n * idx[7]);
+  if (idx[8] >= 0) cross_coef(n, coef[2], u, b, frc + n * idx[8]);
+  for (int f = 0; f < 2; f++) {
+    const double *s = f == 0 ? tt : cc;
+    const int o = idx[9 + f];
+    const double c = coef[3 + f];
+    size_t i;
+    if (o < 0) continue;
+    for (i = 0; i < n; i++) {
+      frc[i + n * o] = u[i] * s[i] * c;
+      frc[i + n * (o + 1)] = u[i + n] * s[i] * c;
+      frc[i + n * (o + 2)] = u[i + 2 * n] * s[i] * c;
+    }
+  }
+  if (idx[11] >= 0) {
+    double *o = frc + n * idx[11];
+    size_t i;
+    for (i = 0; i < n; i++) {
+      const int lt = (int)((i / (size_t)p->nr) % (size_t)p->nth);
+      const double o1 = cos(p->colat[lt]), o2 = -sin(p->colat[lt]), o3 = 0.0;
+      const double c = coef[5];
+      o[i] = -(c * (o2 * u[i + 2 * n] - o3 * u[i + n]));
+      o[i + n] = -(c * (o3 * u[i] - o1 * u[i + 2 * n]));
+      o[i + 2 * n] = -(c * (o1 * u[i + n] - o2 * u[i]));
+    }
+  }
+}

@@ -116,6 +116,18 @@ void orc_fft_fwd_to_rtp(const orc_plan *p, int ncomp, const double *v_rtp,
 void orc_pole_backward(const orc_plan *p, int nv, int ns, const double *WS_rj,
                        double *v_pole, int center);
 
+/* Grid-space products of the MHD step (nonlinear_terms_in_rtp,
+ * MS/cal_nonlinear_sph_MHD.f90:64-128) on field-major rtp buffers.
+ * idx[12]: first component (0-based, -1 = absent/off) of the inputs velo,
+ * vort, magne, current, temp, light and the outputs m_advect = c w x u,
+ * lorentz = c J x B, vp_induct = c u x B, h_flux = c u T, c_flux = c u C,
+ * coriolis = -c (omega x u), omega = (cos theta, -sin theta, 0).
+ * coef[6] in the same order as the outputs.  Pinned against Cartesian cross
+ * products (numpy) of the same vectors; see tests/test_oracle_nonlinear.py. */
+void orc_nonlinear_rtp(const orc_plan *p, const int idx[12],
+                       const double coef[6], int ncomp_fld, const double *fld,
+                       int ncomp_frc, double *frc);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus

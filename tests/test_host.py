@@ -144,6 +144,7 @@ def test_fortran_binding_links_and_validates(tmp_path):
       use kemorin_sht_c
       implicit none
       type(sht_desc) :: d
+      type(sht_nonlinear_desc) :: nl
       type(c_ptr) :: plan
       real(c_double), target :: r(4)
       integer(c_int) :: rc
@@ -159,6 +160,11 @@ def test_fortran_binding_links_and_validates(tmp_path):
       d%flags = 0
       rc = sht_plan_create(d, plan)
       write(*, '(a,i0)') 'rc=', rc
+      nl%i_velo = 0
+      nl%i_coriolis = 0
+      rc = sht_nonlinear_rtp(c_null_ptr, nl, 3_c_int, c_null_ptr,       &
+     &                       3_c_int, c_null_ptr)
+      write(*, '(a,i0)') 'nl=', rc
       end program drv
 """)
     exe = tmp_path / "drv"
@@ -169,6 +175,7 @@ def test_fortran_binding_links_and_validates(tmp_path):
     assert r.returncode == 0, r.stderr
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert "rc=1" in out.stdout, out.stdout + out.stderr
+    assert "nl=1" in out.stdout, out.stdout + out.stderr
 
 
 # --- the split GEMMs' hand-counted waits (checker of the generated code) ---
