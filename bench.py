@@ -104,10 +104,12 @@ def run_nonlinear(args, torch):
                       "data": "synthetic (seeded Gaussian fields)",
                       "config": {"workload": f"{args.config}: Nr={cfg.nr} Ntheta={cfg.nth} "
                                              f"Nphi={cfg.nphi}, 14 input and 18 output components"},
-                      "roofline": {"bound": "hbm", "achieved": nbytes / ms / 1e6,
-                                   "peak": 8000.0, "unit": "GB/s",
+                      "roofline": {"kernel": "ksht::k_nonlinear<2>", "bound": "hbm",
+                                   "achieved": nbytes / ms / 1e6, "peak": 8000.0, "unit": "GB/s",
                                    "frac": nbytes / ms / 1e6 / 8000.0,
-                                   "algorithmic_bytes": nbytes}}))
+                                   "algorithmic_bytes": nbytes,
+                                   "traffic": (hbm_traffic("products", "ksht::k_nonlinear<2>")
+                                               if args.config == "cfg2" else None)}}))
     plan.close()
 
 
