@@ -279,7 +279,32 @@ __global__ void __launch_bounds__(kSliceThreads) k_bslice(
     const int d0 = 2 * i0, nd_run = min(2 * ni, n - d0) * ncomp;
     const int nall = nrad * nd_run;
     __syncthreads();
-    if (sa.s_nd == 1 && sa.s_j == ncomp) {
+    if (sa.s_nd == 1 && sa.s_j == ncomp && nd_run <= 4 * kSliceThreads && nrad <= kBsRadii) {
+      // components contiguous within a degree's record and a run of at
+      // most 4 doubles per thread (ncomp <= 8): the loads of every radius
+      // are issued before any LDS store, so up to 16 per thread are in
+      // flight together
+      double val[kBsRadii][4];
+#pragma unroll
+      for (int rr = 0; rr < kBsRadii; rr++) {
+        const int kr = min(k0 + (rr < nrad ? rr : 0), nr - 1);  // radii >= nr read nr - 1
+        const double *rs = src + (long long)kr * sa.s_k + (long long)d0 * ncomp;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int e = tid + u * kSliceThreads;
+          val[rr][u] = rs[e < nd_run ? e : 0];
+        }
+      }
+#pragma unroll
+      for (int rr = 0; rr < kBsRadii; rr++) {
+        const bool live = k0 + rr < nr;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int e = tid + u * kSliceThreads;
+          if (rr < nrad && e < nd_run) run[rr * 128 * ncomp + e] = live ? val[rr][u] : 0.0;
+        }
+      }
+    } else if (sa.s_nd == 1 && sa.s_j == ncomp) {
       // components contiguous within a degree's record: a radius's run is
       // nd_run consecutive doubles, staged radius by radius without any
       // index division
