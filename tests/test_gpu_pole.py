@@ -67,6 +67,23 @@ def test_pole_backward_matches_oracle(torch_cuda, case):
         got = vp.cpu().numpy()
     finally:
         plan.close()
+    if L == 511:
+        # launch time at the largest shell (HIP events on the stream the plan
+        # follows, torch's current stream), reported only
+        plan = SphTransPlan(cfg, max_ncomp=ncomp, precision=prec)
+        try:
+            spd = t.tensor(sp, device="cuda")
+            vp2 = t.tensor(init, device="cuda")
+            plan.pole_b_transform(nv, ns, spd, vp2, with_center=center)
+            e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(20):
+                plan.pole_b_transform(nv, ns, spd, vp2, with_center=center)
+            e1.record()
+            e1.synchronize()
+            _say(f"    pole transform: {e0.elapsed_time(e1) / 20 * 1e3:.1f} us per launch")
+        finally:
+            plan.close()
     err = _rel(got, ref, ncomp)
     diff = int(np.count_nonzero(got != ref))
     _say(f"    rel L2 {err:.2e}, entries differing from the oracle: {diff}")
