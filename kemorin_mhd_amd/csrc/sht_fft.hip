@@ -941,10 +941,10 @@ int fft_plan_init(sht_plan *p) {
     return SHT_ERR_UNSUPPORTED;
   }
   fp.pad = E;
+  // every pass costs the same whatever F is (its loops are sized by E), so
+  // a work-group takes as many lines as the LDS holds
   if (F > 32) F = 32;
-  int f2 = 1;
-  while (f2 * 2 <= F) f2 *= 2;
-  fp.lines_per_wg = 2 * f2;
+  fp.lines_per_wg = 2 * F;
   std::vector<double2> tw;
   const long double kPi = 3.14159265358979323846264338327950288L;
   int Ns = 1;
@@ -1035,7 +1035,7 @@ int fft_plan_init(sht_plan *p) {
       ms *= q.R;
     }
   }
-  const int lds = (int)(sizeof(double2) * (size_t)f2 * fp.line_elems);
+  const int lds = (int)(sizeof(double2) * (size_t)(fp.lines_per_wg / 2) * fp.line_elems);
   bool gen = false;
   for (int R : fac) gen |= !(R == 2 || R == 3 || R == 4 || R == 8);
   p->fft_generic = gen;
