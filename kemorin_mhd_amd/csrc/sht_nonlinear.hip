@@ -134,6 +134,10 @@ extern "C" int sht_nonlinear_rtp(sht_plan *p, const sht_nonlinear_desc *d, int n
     set_error("sht_nonlinear_rtp: null argument");
     return SHT_ERR_ARG;
   }
+  if (ncomp_fld < 1 || ncomp_frc < 1) {
+    set_error("sht_nonlinear_rtp: component counts %d, %d must be positive", ncomp_fld, ncomp_frc);
+    return SHT_ERR_ARG;
+  }
   const NlArgs a = {{d->i_velo, d->i_vort, d->i_magne, d->i_current, d->i_temp, d->i_light,
                      d->i_m_advect, d->i_lorentz, d->i_vp_induct, d->i_h_flux, d->i_c_flux,
                      d->i_coriolis},
@@ -147,7 +151,7 @@ extern "C" int sht_nonlinear_rtp(sht_plan *p, const sht_nonlinear_desc *d, int n
   bool used[6] = {false, false, false, false, false, false};
   for (int o = 0; o < 6; o++) {
     const int io = a.idx[6 + o];
-    if (io < -1 || io + 3 > ncomp_frc) {
+    if (io < -1 || (io >= 0 && io + 3 > ncomp_frc)) {
       set_error("sht_nonlinear_rtp: output %s at component %d outside -1 or 0..%d", names[6 + o],
                 io, ncomp_frc - 3);
       return SHT_ERR_ARG;

@@ -86,6 +86,12 @@ def test_nonlinear_rejects_missing_inputs(torch_cuda):
             plan.nonlinear_terms_in_rtp({"velo": 0, "m_advect": 0}, COEF, 3, f, 3, o)
         with pytest.raises(Exception, match="outside"):
             plan.nonlinear_terms_in_rtp({"velo": 0, "coriolis": 1}, COEF, 3, f, 3, o)
+        with pytest.raises(Exception, match="positive"):
+            plan.nonlinear_terms_in_rtp({"velo": 0}, COEF, 0, f, 3, o)
+        # every term off: a one-component output is accepted and left alone
+        o1 = t.full((cfg.n_rtp(1),), 3.0, dtype=t.float64, device="cuda")
+        plan.nonlinear_terms_in_rtp({}, COEF, 3, f, 1, o1)
+        assert bool((o1 == 3.0).all())
     finally:
         plan.close()
 

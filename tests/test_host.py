@@ -89,6 +89,9 @@ def test_plan_create_rejects_null_and_nonpositive_radius():
 def test_calls_on_null_plan_fail_cleanly():
     L = sht.lib()
     assert L.sht_backward(None, 1, 0, None, None) != 0
+    assert L.sht_pole_backward(None, 1, 0, None, None, 1) != 0
+    assert L.sht_backward_with_poles(None, 1, 0, None, None, None, 0) != 0
+    assert L.sht_nonlinear_rtp(None, None, 3, None, 3, None) != 0
     assert L.sht_plan_destroy(None) == 0 or True  # destroying NULL is a no-op
 
 
