@@ -11,9 +11,10 @@
 // (x, y, z) = (v_theta, v_phi, v_r) at phi = 0 in the north and
 // (-v_theta, v_phi, -v_r) in the south.
 //
-// The work is O(Nr L ncomp) and latency-bound: one thread per output,
-// summing over l in the reference's order with contraction off, so the
-// result is the reference's operation sequence on the same tables.
+// The work is O(Nr L ncomp) and latency-bound: one wave per output, whose
+// lanes form the per-degree products and whose lane 0 adds them in the
+// reference's order with contraction off, so the result is the reference's
+// operation sequence on the same tables.
 #include <cmath>
 #include <vector>
 
@@ -63,8 +64,30 @@ int pole_tables_init(sht_plan *p) {
   return SHT_OK;
 }
 
-// One thread per output row i of component nd.  j0, jm, jp: spectral index
-// of (l, m) = (0, 0), (1, -1), (1, 1) (degrees ascending within an order).
+// Sum over degrees l = lo..hi of term(l), in ascending l: the lanes of the
+// (single-wave) work-group form the terms of a 64-degree chunk together, and
+// lane 0 adds them in order with add(acc, t), so the loads run in parallel
+// while the rounding sequence stays the sequential one.
+template <class Term, class Add>
+__device__ __forceinline__ double ordered_sum(double acc, int lo, int hi, double *term, Term tf,
+                                              Add add) {
+  const int lane = threadIdx.x;
+  for (int l0 = lo; l0 <= hi; l0 += 64) {
+    const int l = l0 + lane;
+    if (l <= hi) term[lane] = tf(l);
+    __syncthreads();
+    if (lane == 0) {
+      const int n = min(64, hi - l0 + 1);
+      for (int q = 0; q < n; q++) acc = add(acc, term[q]);
+    }
+    __syncthreads();
+  }
+  return acc;
+}
+
+// One wave per output row i (blockIdx.x) of component nd (blockIdx.y).
+// j0, jm, jp: spectral index of (l, m) = (0, 0), (1, -1), (1, 1) (degrees
+// ascending within an order).
 __global__ void __launch_bounds__(64) k_pole_bwd(int L, int nr, int nv, int ns, int center,
                                                  int j0, int jm, int jp, long long jmax,
                                                  const double *__restrict__ sp,
@@ -72,15 +95,15 @@ __global__ void __launch_bounds__(64) k_pole_bwd(int L, int nr, int nv, int ns, 
                                                  const double *__restrict__ a_r,
                                                  double *__restrict__ vp) {
 #pragma clang fp contract(off)
+  __shared__ double term[64];
   const int ncomp = 3 * nv + ns, npole = 2 * nr + 1;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, nd = blockIdx.y;
-  if (i >= npole) return;
+  const int i = blockIdx.x, nd = blockIdx.y, lane = threadIdx.x;
   double *o = vp + i + (long long)npole * nd;
   const long long sj = ncomp;  // stride of one degree/order step
   if (i == 2 * nr) {
     // centre (radius index 0): the degree-1 poloidal terms; scalars and
     // every centre entry without a centre are left as they are
-    if (!center || nd >= 3 * nv) return;
+    if (!center || nd >= 3 * nv || lane != 0) return;
     const int f = nd / 3, c = nd % 3;
     const double a = a_r[0];
     const double *pol = sp + 3 * f;
@@ -95,10 +118,13 @@ __global__ void __launch_bounds__(64) k_pole_bwd(int L, int nr, int nv, int ns, 
   const double *P = tab + (north ? 0 : L + 1);
   const double *dP = tab + 2 * (L + 1) + (north ? 0 : L + 1);
   const double *s = sp + (long long)ncomp * jmax * k;
+  auto plus = [](double acc, double t) { return acc + t; };
+  auto minus = [](double acc, double t) { return acc - t; };
+  auto sgn = [north](double acc, double t) { return north ? acc + t : acc - t; };
   double acc = 0.0;
   if (nd >= 3 * nv) {
-    for (int l = 0; l <= L; l++) acc = acc + s[nd + sj * (j0 + l)] * P[l];
-    *o = acc;
+    acc = ordered_sum(acc, 0, L, term, [&](int l) { return s[nd + sj * (j0 + l)] * P[l]; }, plus);
+    if (lane == 0) *o = acc;
     return;
   }
   const int f = nd / 3, c = nd % 3;
@@ -107,33 +133,26 @@ __global__ void __launch_bounds__(64) k_pole_bwd(int L, int nr, int nv, int ns, 
   if (c == 2) {
     // z: order 0 of the poloidal term times l(l+1) (1/2 at l = 0, the
     // normalisation table's value, SP/schmidt_poly_on_rtm_grid.f90:109)
-    for (int l = 0; l <= L; l++) {
+    acc = ordered_sum(acc, 0, L, term, [&](int l) {
       const double g3 = l == 0 ? 0.5 : (double)(l * (l + 1));
-      const double t = pol[sj * (j0 + l)] * P[l] * g3;
-      acc = north ? acc + t : acc - t;
-    }
-    *o = acc * a * a;
+      return pol[sj * (j0 + l)] * P[l] * g3;
+    }, sgn);
+    if (lane == 0) *o = acc * a * a;
   } else if (c == 0) {
-    for (int l = 1; l <= L; l++) acc = acc + tor[sj * (jm + l - 1)] * dP[l];
-    for (int l = 1; l <= L; l++) {
-      const double t = dpol[sj * (jp + l - 1)] * dP[l];
-      acc = north ? acc + t : acc - t;
-    }
-    *o = acc * a;
+    acc = ordered_sum(acc, 1, L, term, [&](int l) { return tor[sj * (jm + l - 1)] * dP[l]; }, plus);
+    acc = ordered_sum(acc, 1, L, term, [&](int l) { return dpol[sj * (jp + l - 1)] * dP[l]; }, sgn);
+    if (lane == 0) *o = acc * a;
   } else {
-    for (int l = 1; l <= L; l++) {
-      const double t = dpol[sj * (jm + l - 1)] * dP[l];
-      acc = north ? acc + t : acc - t;
-    }
-    for (int l = 1; l <= L; l++) acc = acc - tor[sj * (jp + l - 1)] * dP[l];
-    *o = acc * a;
+    acc = ordered_sum(acc, 1, L, term, [&](int l) { return dpol[sj * (jm + l - 1)] * dP[l]; }, sgn);
+    acc = ordered_sum(acc, 1, L, term, [&](int l) { return tor[sj * (jp + l - 1)] * dP[l]; }, minus);
+    if (lane == 0) *o = acc * a;
   }
 }
 
 static int launch_pole_bwd(sht_plan *p, int nv, int ns, const double *sp, double *vp,
                            int center) {
   const int ncomp = 3 * nv + ns, npole = 2 * p->nr + 1, L = p->L;
-  dim3 grid((npole + 63) / 64, ncomp);
+  dim3 grid(npole, ncomp);
   hipLaunchKernelGGL(k_pole_bwd, grid, dim3(64), 0, p->stream, L, p->nr, nv, ns, center,
                      p->lstack[L], p->lstack[L - 1], p->lstack[L + 1], (long long)p->jmax, sp,
                      p->t.pole, p->t.a_r, vp);
