@@ -119,11 +119,11 @@ def run_shells(args, torch):
     shell in both directions.  A step is one backward and one forward
     transform of every shell: as a loop of per-shell calls on one stream, and
     as one launch of the shells' captured graph."""
-    from kemorin_mhd_amd import (SHT_PREC_FP64, SHT_PREC_SPLIT_INT8, ShellBatch, ShellConfig,
-                                 SphTransPlan)
-    rng = np.random.default_rng(20261202)
-    Ls = [int(x) for x in rng.choice(np.arange(63, 384, 2), size=args.shells)]
-    nrs = [int(x) for x in rng.integers(33, 130, size=args.shells)]
+    from kemorin_mhd_amd import (SHT_PREC_FP64, SHT_PREC_SPLIT_INT8, ShellBatch, SphTransPlan,
+                                 batched_shell_draw)
+    shells = batched_shell_draw(args.shells)
+    Ls = [c.L for c in shells]
+    nrs = [c.nr for c in shells]
     precs = ["fp64", "split"] if args.precision in ("auto", "both") else [args.precision]
     gen = torch.Generator(device="cuda")
     gen.manual_seed(20261202)
@@ -141,8 +141,7 @@ def run_shells(args, torch):
     for prec in precs:
         plans, bufs = [], []
         with torch.cuda.stream(stream):
-            for L, nr in zip(Ls, nrs):
-                c = ShellConfig(L, nr, 3 * (L + 1) // 2, 3 * (L + 1))
+            for c in shells:
                 p = SphTransPlan(c, max_ncomp=4, precision=SHT_PREC_SPLIT_INT8 if prec == "split"
                                  else SHT_PREC_FP64, n_slices=args.slices)
                 p.set_stream(stream.cuda_stream)

@@ -70,39 +70,47 @@ static FourierAddr rtp_fourier_addr(const sht_plan *p, int ncomp) {
   return xa;
 }
 
-static int leg_bwd(sht_plan *p, int nv, int ns, const double *sp, SpecAddr sa, double *out,
-                   GridAddr ga) {
+static int leg_bwd(sht_plan *p, const Workspace &ws, int nv, int ns, const double *sp,
+                   SpecAddr sa, double *out, GridAddr ga) {
   StageTimer tm(p, 0);
-  if (p->precision == SHT_PREC_SPLIT_INT8) return launch_leg_bwd_split(p, nv, ns, sp, sa, out, ga);
+  if (p->precision == SHT_PREC_SPLIT_INT8)
+    return launch_leg_bwd_split(p, ws, nv, ns, sp, sa, out, ga);
   return launch_leg_bwd(p, nv, ns, sp, sa, out, ga);
 }
-static int leg_fwd(sht_plan *p, int nv, int ns, const double *in, GridAddr ga, double *sp,
-                   SpecAddr sa) {
+static int leg_fwd(sht_plan *p, const Workspace &ws, int nv, int ns, const double *in,
+                   GridAddr ga, double *sp, SpecAddr sa) {
   StageTimer tm(p, 3);
-  if (p->precision == SHT_PREC_SPLIT_INT8) return launch_leg_fwd_split(p, nv, ns, in, ga, sp, sa);
+  if (p->precision == SHT_PREC_SPLIT_INT8)
+    return launch_leg_fwd_split(p, ws, nv, ns, in, ga, sp, sa);
   return launch_leg_fwd(p, nv, ns, in, ga, sp, sa);
+}
+
+static int backward_ws(sht_plan *p, const Workspace &ws, int nv, int ns, const double *WS_rj,
+                       double *v_rtp) {
+  const int ncomp = 3 * nv + ns;
+  int rc = leg_bwd(p, ws, nv, ns, WS_rj, spec_addr(p, ncomp), ws.X, x_grid_addr(p));
+  if (rc) return rc;
+  StageTimer tm(p, 1);
+  return launch_fft_bwd(p, ncomp, ws.X, x_fourier_addr(p), v_rtp);
 }
 
 extern "C" int sht_backward(sht_plan *p, int nv, int ns, const double *WS_rj, double *v_rtp) {
   int rc = check_call(p, nv, ns, WS_rj, v_rtp, "sht_backward");
   if (rc) return rc;
-  const int ncomp = 3 * nv + ns;
-  rc = leg_bwd(p, nv, ns, WS_rj, spec_addr(p, ncomp), p->X, x_grid_addr(p));
-  if (rc) return rc;
-  StageTimer tm(p, 1);
-  return launch_fft_bwd(p, ncomp, p->X, x_fourier_addr(p), v_rtp);
+  return backward_ws(p, plan_workspace(p), nv, ns, WS_rj, v_rtp);
 }
 
 extern "C" int sht_forward(sht_plan *p, int nv, int ns, const double *v_rtp, double *WR_rj) {
   int rc = check_call(p, nv, ns, v_rtp, WR_rj, "sht_forward");
   if (rc) return rc;
   const int ncomp = 3 * nv + ns;
+  const Workspace ws = plan_workspace(p);
   {
     StageTimer tm(p, 2);
-    rc = launch_fft_fwd(p, ncomp, v_rtp, p->X, x_fourier_addr(p));
+    rc = launch_fft_fwd(p, ncomp, v_rtp, ws.X, x_fourier_addr(p));
   }
   if (rc) return rc;
-  return leg_fwd(p, nv, ns, p->X, x_grid_addr(p), WR_rj, spec_addr(p, ncomp));
+  return leg_fwd(p, ws, nv, ns, ws.X, x_grid_addr(p), WR_rj, spec_addr(p, ncomp));
 }
 
 static int host_call(sht_plan *p, int nv, int ns, const double *in, double *out, bool bwd) {
@@ -142,7 +150,8 @@ extern "C" int sht_legendre_backward(sht_plan *p, int nv, int ns, const double *
   int rc = check_call(p, nv, ns, WR_rlm, WS_rtm, "sht_legendre_backward");
   if (rc) return rc;
   const int ncomp = 3 * nv + ns;
-  return leg_bwd(p, nv, ns, WR_rlm, spec_addr(p, ncomp), WS_rtm, rtm_grid_addr(p, ncomp));
+  return leg_bwd(p, plan_workspace(p), nv, ns, WR_rlm, spec_addr(p, ncomp), WS_rtm,
+                 rtm_grid_addr(p, ncomp));
 }
 
 extern "C" int sht_legendre_forward(sht_plan *p, int nv, int ns, const double *WR_rtm,
@@ -150,7 +159,8 @@ extern "C" int sht_legendre_forward(sht_plan *p, int nv, int ns, const double *W
   int rc = check_call(p, nv, ns, WR_rtm, WS_rlm, "sht_legendre_forward");
   if (rc) return rc;
   const int ncomp = 3 * nv + ns;
-  return leg_fwd(p, nv, ns, WR_rtm, rtm_grid_addr(p, ncomp), WS_rlm, spec_addr(p, ncomp));
+  return leg_fwd(p, plan_workspace(p), nv, ns, WR_rtm, rtm_grid_addr(p, ncomp), WS_rlm,
+                 spec_addr(p, ncomp));
 }
 
 extern "C" int sht_fft_backward(sht_plan *p, int ncomp, const double *WR_rtp, double *v_rtp) {

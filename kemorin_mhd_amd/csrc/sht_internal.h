@@ -161,6 +161,21 @@ struct StageTimer {
 }  // namespace ksht
 
 namespace ksht {
+// Per-call workspace of a whole transform: the field-major Fourier workspace
+// between the Legendre and FFT kernels and, for split-precision plans, the
+// sliced data operand (digits and exponents): the plan's own buffers
+// (plan_workspace).
+struct Workspace {
+  double *X;
+  signed char *dig;
+  int *dexp;
+};
+Workspace plan_workspace(const sht_plan *p);
+// bytes of the split engine's data-operand digits and exponents for a call of
+// ncomp components in one direction
+void split_ws_bytes(const sht_plan *p, int ncomp, bool backward, long long *dig_bytes,
+                    long long *dexp_bytes);
+
 // Legendre launchers (sht_legendre.hip)
 int launch_leg_bwd(sht_plan *p, int nv, int ns, const double *sp, SpecAddr sa,
                    double *out, GridAddr ga);
@@ -184,8 +199,8 @@ int pole_tables_init(sht_plan *p);
 // split precision (sht_split.hip)
 int split_init(sht_plan *p);
 void split_free(sht_plan *p);
-int launch_leg_bwd_split(sht_plan *p, int nv, int ns, const double *sp,
+int launch_leg_bwd_split(sht_plan *p, const Workspace &ws, int nv, int ns, const double *sp,
                          SpecAddr sa, double *out, GridAddr ga);
-int launch_leg_fwd_split(sht_plan *p, int nv, int ns, const double *in,
+int launch_leg_fwd_split(sht_plan *p, const Workspace &ws, int nv, int ns, const double *in,
                          GridAddr ga, double *sp, SpecAddr sa);
 }  // namespace ksht

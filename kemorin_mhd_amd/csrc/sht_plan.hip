@@ -246,17 +246,25 @@ extern "C" int sht_plan_create(const sht_desc *d, sht_plan **out) {
   }
   *out = nullptr;
   const int L = d->l_truncation, nr = d->nidx_r, nth = d->nidx_theta, nphi = d->nidx_phi;
-  if (L < 1 || nr < 1 || nth < 2 || nphi < 2 * L + 2 || (nphi & 1) || !d->radius ||
-      d->max_ncomp < 1 || d->max_ncomp > 64 || nth < L + 1) {
-    set_error("sht_plan_create: bad descriptor (L=%d nr=%d nth=%d nphi=%d ncomp=%d): need "
-              "L>=1, nth>=L+1, even nphi>=2L+2, 1<=max_ncomp<=64, radius",
-              L, nr, nth, nphi, d->max_ncomp);
-    return SHT_ERR_ARG;
-  }
-  if (d->precision != SHT_PREC_FP64 && d->precision != SHT_PREC_SPLIT_INT8) {
-    set_error("sht_plan_create: unknown precision %d", d->precision);
-    return SHT_ERR_ARG;
-  }
+  // one message per field, naming it and the value received
+#define BAD(...)                                  \
+  do {                                            \
+    set_error("sht_plan_create: " __VA_ARGS__);   \
+    return SHT_ERR_ARG;                           \
+  } while (0)
+  if (L < 1) BAD("l_truncation = %d: need >= 1", L);
+  if (nr < 1) BAD("nidx_r = %d: need >= 1", nr);
+  if (nth < L + 1 || nth < 2) BAD("nidx_theta = %d: need >= max(2, L+1) = %d", nth, L + 1 > 2 ? L + 1 : 2);
+  if (nphi < 2 * L + 2 || (nphi & 1)) BAD("nidx_phi = %d: need even and >= 2L+2 = %d", nphi, 2 * L + 2);
+  if (!d->radius) BAD("radius: null pointer");
+  if (d->max_ncomp < 1 || d->max_ncomp > 64) BAD("max_ncomp = %d: need 1..64", d->max_ncomp);
+  if (d->precision != SHT_PREC_FP64 && d->precision != SHT_PREC_SPLIT_INT8)
+    BAD("precision = %d: need SHT_PREC_FP64 (0) or SHT_PREC_SPLIT_INT8 (1)", d->precision);
+  if (d->precision == SHT_PREC_SPLIT_INT8 && d->n_slices != 0 &&
+      (d->n_slices < 4 || d->n_slices > 7))
+    BAD("n_slices = %d: need 0 (default) or 4..7 in split mode", d->n_slices);
+  if (d->flags != 0) BAD("flags = %d: reserved, pass 0", d->flags);
+#undef BAD
   for (int k = 0; k < nr; k++)
     if (!(d->radius[k] > 0.0)) {
       set_error("sht_plan_create: radius[%d] must be positive", k);

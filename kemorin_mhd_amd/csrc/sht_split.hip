@@ -1275,6 +1275,20 @@ void split_free(sht_plan *p) {
   p->split = nullptr;
 }
 
+void split_ws_bytes(const sht_plan *p, int ncomp, bool backward, long long *dig_bytes,
+                    long long *dexp_bytes) {
+  const SplitState *st = (const SplitState *)p->split;
+  const long long groups =
+      backward ? st->nchb * ncomp : (long long)p->nslot * st->nchf * 2 * ncomp;
+  *dig_bytes = groups * st->S * (p->nr_p / 16) * 1024;
+  *dexp_bytes = groups * p->nr_p * 4;
+}
+
+Workspace plan_workspace(const sht_plan *p) {
+  const SplitState *st = (const SplitState *)p->split;
+  return Workspace{p->X, st ? st->dig : nullptr, st ? st->dexp : nullptr};
+}
+
 static SplitDims split_dims(const sht_plan *p, int ncomp, int nblk) {
   SplitDims d;
   d.L = p->L;
@@ -1305,8 +1319,8 @@ static int component_groups(int nv, int ns, GroupLaunch g[3]) {
 }
 
 template <int S>
-static int bwd_split_S(sht_plan *p, int nv, int ns, const double *spec, SpecAddr sa, double *out,
-                       GridAddr ga) {
+static int bwd_split_S(sht_plan *p, const Workspace &ws, int nv, int ns, const double *spec,
+                       SpecAddr sa, double *out, GridAddr ga) {
   SplitState *st = (SplitState *)p->split;
   const int ncomp = 3 * nv + ns;
   // radii per work-group: 4, unless the staging [nrad][128][ncomp] doubles
@@ -1314,8 +1328,8 @@ static int bwd_split_S(sht_plan *p, int nv, int ns, const double *spec, SpecAddr
   const int nrad = ncomp <= 16 ? 4 : ncomp <= 32 ? 2 : 1;
   hipLaunchKernelGGL(k_bslice<S>, dim3(p->nr_p / nrad, p->nslot), dim3(kSliceThreads),
                      sizeof(double) * nrad * 128 * ncomp, p->stream,
-                     p->L, p->nr, p->nr_p, nrad, nv, ncomp, spec, sa, p->t.jst, st->cb, st->dig,
-                     st->dexp);
+                     p->L, p->nr, p->nr_p, nrad, nv, ncomp, spec, sa, p->t.jst, st->cb, ws.dig,
+                     ws.dexp);
   KSHT_HIP(hipGetLastError());
   SplitTabs tb{st->b_off, st->b_eoff, st->bP, st->bdP, nullptr, st->beP, st->bedP, nullptr};
   GroupLaunch g[3];
@@ -1327,7 +1341,7 @@ static int bwd_split_S(sht_plan *p, int nv, int ns, const double *spec, SpecAddr
     const dim3 grid((unsigned)nblocks), blk(kGemmThreads);
 #define KSHT_BWD(NB, VEC)                                                                      \
   hipLaunchKernelGGL((k_leg_bwd_split<S, NB, VEC>), grid, blk, bwd_lds<S>(NB), p->stream, d, \
-                     g[i].nd_first, tb, st->cb, st->dig, st->dexp, out, ga, p->t.a_r,        \
+                     g[i].nd_first, tb, st->cb, ws.dig, ws.dexp, out, ga, p->t.a_r,          \
                      p->t.asin_t)
     if (g[i].vec) {
       StageTimer tk(p, 5);
@@ -1345,13 +1359,13 @@ static int bwd_split_S(sht_plan *p, int nv, int ns, const double *spec, SpecAddr
 }
 
 template <int S>
-static int fwd_split_S(sht_plan *p, int nv, int ns, const double *in, GridAddr ga, double *sp,
-                       SpecAddr sa) {
+static int fwd_split_S(sht_plan *p, const Workspace &ws, int nv, int ns, const double *in,
+                       GridAddr ga, double *sp, SpecAddr sa) {
   SplitState *st = (SplitState *)p->split;
   const int ncomp = 3 * nv + ns;
   hipLaunchKernelGGL(k_fslice<S>, dim3((p->nr_p / 16) * p->nslot), dim3(kSliceThreads), 0,
                      p->stream, p->nr, p->nr_p, p->nth, p->nth_h, st->nchf, ncomp, in, ga,
-                     p->t.weight, st->dig, st->dexp);
+                     p->t.weight, ws.dig, ws.dexp);
   KSHT_HIP(hipGetLastError());
   SplitTabs tb{st->f_off, st->f_eoff, st->fP, st->fdP, st->fPa, st->feP, st->fedP, st->fePa};
   const int ne_max = (p->L + 2) / 2;
@@ -1364,7 +1378,7 @@ static int fwd_split_S(sht_plan *p, int nv, int ns, const double *in, GridAddr g
     const dim3 grid((unsigned)nblocks), blk(kGemmThreads);
 #define KSHT_FWD(NB, VEC)                                                                      \
   hipLaunchKernelGGL((k_leg_fwd_split<S, NB, VEC>), grid, blk, fwd_lds<S>(NB), p->stream, d, \
-                     g[i].nd_first, tb, st->dig, st->dexp, sp, sa, p->t.jst, p->t.ipad,      \
+                     g[i].nd_first, tb, ws.dig, ws.dexp, sp, sa, p->t.jst, p->t.ipad,        \
                      p->t.radius)
     if (g[i].vec) {
       StageTimer tk(p, 6);
@@ -1381,8 +1395,8 @@ static int fwd_split_S(sht_plan *p, int nv, int ns, const double *in, GridAddr g
   return SHT_OK;
 }
 
-int launch_leg_bwd_split(sht_plan *p, int nv, int ns, const double *sp, SpecAddr sa, double *out,
-                         GridAddr ga) {
+int launch_leg_bwd_split(sht_plan *p, const Workspace &ws, int nv, int ns, const double *sp,
+                         SpecAddr sa, double *out, GridAddr ga) {
   if (!p->split) {
     set_error("split precision: plan was not created in split mode");
     return SHT_ERR_STATE;
@@ -1393,15 +1407,15 @@ int launch_leg_bwd_split(sht_plan *p, int nv, int ns, const double *sp, SpecAddr
     return SHT_ERR_ARG;
   }
   switch (((SplitState *)p->split)->S) {
-    case 4: return bwd_split_S<4>(p, nv, ns, sp, sa, out, ga);
-    case 5: return bwd_split_S<5>(p, nv, ns, sp, sa, out, ga);
-    case 6: return bwd_split_S<6>(p, nv, ns, sp, sa, out, ga);
-    default: return bwd_split_S<7>(p, nv, ns, sp, sa, out, ga);
+    case 4: return bwd_split_S<4>(p, ws, nv, ns, sp, sa, out, ga);
+    case 5: return bwd_split_S<5>(p, ws, nv, ns, sp, sa, out, ga);
+    case 6: return bwd_split_S<6>(p, ws, nv, ns, sp, sa, out, ga);
+    default: return bwd_split_S<7>(p, ws, nv, ns, sp, sa, out, ga);
   }
 }
 
-int launch_leg_fwd_split(sht_plan *p, int nv, int ns, const double *in, GridAddr ga, double *sp,
-                         SpecAddr sa) {
+int launch_leg_fwd_split(sht_plan *p, const Workspace &ws, int nv, int ns, const double *in,
+                         GridAddr ga, double *sp, SpecAddr sa) {
   if (!p->split) {
     set_error("split precision: plan was not created in split mode");
     return SHT_ERR_STATE;
@@ -1412,10 +1426,10 @@ int launch_leg_fwd_split(sht_plan *p, int nv, int ns, const double *in, GridAddr
     return SHT_ERR_ARG;
   }
   switch (((SplitState *)p->split)->S) {
-    case 4: return fwd_split_S<4>(p, nv, ns, in, ga, sp, sa);
-    case 5: return fwd_split_S<5>(p, nv, ns, in, ga, sp, sa);
-    case 6: return fwd_split_S<6>(p, nv, ns, in, ga, sp, sa);
-    default: return fwd_split_S<7>(p, nv, ns, in, ga, sp, sa);
+    case 4: return fwd_split_S<4>(p, ws, nv, ns, in, ga, sp, sa);
+    case 5: return fwd_split_S<5>(p, ws, nv, ns, in, ga, sp, sa);
+    case 6: return fwd_split_S<6>(p, ws, nv, ns, in, ga, sp, sa);
+    default: return fwd_split_S<7>(p, ws, nv, ns, in, ga, sp, sa);
   }
 }
 

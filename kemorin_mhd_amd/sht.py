@@ -430,6 +430,17 @@ def gaussian(seed: int, n: int) -> np.ndarray:
     return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
 
 
+def batched_shell_draw(n: int = 64, seed: int = 20261202):
+    """Seeded uneven shells of the batched-shell workload: L drawn from the
+    odd values in 63..383, Nr from 33..129, Ntheta = 3(L+1)/2 Gauss points
+    and Nphi = 2 Ntheta longitudes (1 vector + 1 scalar per shell each way).
+    Returns a list of ShellConfig."""
+    rng = np.random.default_rng(seed)
+    Ls = [int(x) for x in rng.choice(np.arange(63, 384, 2), size=n)]
+    nrs = [int(x) for x in rng.integers(33, 130, size=n)]
+    return [ShellConfig(L, nr, 3 * (L + 1) // 2, 3 * (L + 1)) for L, nr in zip(Ls, nrs)]
+
+
 def rlm_l_m(L: int):
     """(l, m) of each j in the single-rank rlm order (m = -L..L, l = |m|..L)."""
     ls, ms = [], []

@@ -41,6 +41,7 @@ def lib():
         L.ref_get_table.argtypes = [ctypes.c_int, _dp]
         L.ref_leg_bwd.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.ref_leg_fwd.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.ref_center.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -126,3 +127,12 @@ class RefPlan:
         WS = np.zeros(ncomp * self.jmax * self.nr)
         lib().ref_leg_fwd(nv, ns, _p(WR), _p(WS))
         return WS
+
+    def center(self, nv, ns, sp, v_pole):
+        """The reference's centre row (SS/schmidt_b_trans_at_center.f90)
+        applied to a copy of v_pole (ncomp x (2 nr + 1), component-major)."""
+        WR = np.ascontiguousarray(sp, dtype=np.float64).ravel().copy()
+        out = np.array(v_pole, dtype=np.float64).ravel().copy()
+        assert out.size == (3 * nv + ns) * (2 * self.nr + 1)
+        lib().ref_center(nv, ns, _p(WR), _p(out))
+        return out

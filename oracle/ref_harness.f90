@@ -26,6 +26,7 @@
       use cal_vr_rtm_by_matmul
       use set_vr_rtm_for_leg_matmul
       use cal_sp_rlm_by_matmul
+      use schmidt_b_trans_at_center
 !
       implicit none
 !
@@ -184,6 +185,47 @@
      &    lstack_h, lstack_e_h, P_h, dP_h, Ps_h, dPs_h)
       deallocate(p, dp, pmn1, pmp1, df)
       end subroutine ref_setup
+!
+!> Centre row of the backward transform at the poles and the centre
+!! (SS/schmidt_b_trans_at_center.f90:45-115) on the tables of ref_setup:
+!! ist_rtm_order_zero / _1s / _1c are the first rlm j of orders 0, -1, +1
+!! (CORE_src/SPH_SPECTR_src/set_special_sph_lm_flags.f90:79-101),
+!! irev_sr_rlm is the identity, the innermost radius has global index 1.
+!! WR: rlm receive buffer ncomp*(jmax*nr); v_pole(2nr+1, ncomp): only the
+!! centre row 2nr+1 is touched (vector components set, scalars kept).
+      subroutine ref_center(nvector, nscalar, WR, v_pole)               &
+     &          bind(C, name='ref_center')
+      integer(c_int), value :: nvector, nscalar
+      real(c_double), intent(in) :: WR(*)
+      real(c_double), intent(inout) :: v_pole(*)
+      integer(kind = kint) :: ncomp, nnod_rlm, nnod_pole, n_WR, j
+      integer(kind = kint) :: ist0, ist1s, ist1c, nvk, nsk
+      integer(kind = kint), allocatable :: irev(:)
+!
+      nvk = nvector
+      nsk = nscalar
+      ncomp = 3*nvk + nsk
+      nnod_rlm = jmax_h * nr_h
+      nnod_pole = 2*nr_h + 1
+      n_WR = ncomp * nnod_rlm
+      ist0 = 0
+      ist1s = 0
+      ist1c = 0
+      do j = jmax_h, 1, -1
+        if(idx_j_h(j,3) .eq.  0) ist0 = j
+        if(idx_j_h(j,3) .eq. -1) ist1s = j
+        if(idx_j_h(j,3) .eq.  1) ist1c = j
+      end do
+      allocate(irev(nnod_rlm))
+      do j = 1, nnod_rlm
+        irev(j) = j
+      end do
+      call schmidt_b_trans_center_vect(ncomp, nvk, ist0, ist1s, ist1c,  &
+     &    nnod_rlm, nnod_pole, 1, a_r_h(1), irev, n_WR, WR, v_pole)
+      call schmidt_b_trans_center_scalar(ncomp, nvk, nsk, nnod_pole,    &
+     &    v_pole)
+      deallocate(irev)
+      end subroutine ref_center
 !
       subroutine ref_release() bind(C, name='ref_release')
       if(allocated(radius_h)) deallocate(radius_h, a_r_h)

@@ -1136,8 +1136,8 @@ void orc_sht_forward(const orc_plan *p, int nv, int ns, const double *v_rtp,
  * (vector); centre: SS/schmidt_b_trans_at_center.f90:45-62 (scalar, leaves
  * the row as it is) and :66-115 (vector).  The vector centre follows that
  * code (x from order -1, y from order +1), not its header comment (which
- * names the cos mode for x); nothing in the reference pins the centre row
- * beyond the code, so its parity is unpinned.  Rows of v_pole: 0..nr-1 north
+ * names the cos mode for x); tests pin the centre row bit for bit against
+ * that routine compiled from its source.  Rows of v_pole: 0..nr-1 north
  * pole, nr..2nr-1 south pole, 2nr the centre (nnod_pole = 2nr + 1,
  * SP/t_spheric_rtp_data.f90:187); v_pole[i + nnod_pole*nd].  The centre row
  * is written for vector components only and only when center != 0

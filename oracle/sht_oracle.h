@@ -111,8 +111,8 @@ void orc_fft_fwd_to_rtp(const orc_plan *p, int ncomp, const double *v_rtp,
  * SS/pole_sph_transform.f90:107-162, SS/schmidt_b_trans_at_pole.f90,
  * SS/schmidt_b_trans_at_center.f90, SP/schmidt_poly_on_rtm_grid.f90:201-266.
  * Pinned: the pole tables by closed forms, the pole rows by the limit of the
- * full-grid backward at phi = 0 and by uniform fields.  The vector centre
- * row: parity unpinned (restated from the code only). */
+ * full-grid backward at phi = 0 and by uniform fields; the centre row bit
+ * for bit against SS/schmidt_b_trans_at_center.f90 compiled from source. */
 void orc_pole_backward(const orc_plan *p, int nv, int ns, const double *WS_rj,
                        double *v_pole, int center);
 

@@ -1,0 +1,144 @@
+"""Both engines against the oracle on the benchmark's own grid.
+
+L=511, Nr=257, Ntheta=768, Nphi=1536: the plan transforms all 257 radii
+(so the radius tiling, including the ragged last tile, is the benchmark's);
+the oracle, in which radii are independent, takes five of them: both
+boundaries, their neighbours and the middle.  Spectra are (l+1)^-1.5 with a
+radial profile 1/2 + sin(pi (r-ri)/(ro-ri)), non-zero at the boundaries so
+that the 1/r and r^2 factors there are exercised.
+
+Bars: FP64 engine within relative L2 1e-12 per component of the oracle,
+split engine (6 INT8 slices) within 1e-11, for the whole backward (2 vector
++ 2 scalar) and forward (3 vector) transforms.  The FFT stages alone (the
+split-form kernels of Nphi = 3 * 512) within 1e-12 per component.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from kemorin_mhd_amd import (SHT_PREC_FP64, SHT_PREC_SPLIT_INT8, ShellConfig, SphTransPlan,
+                             rlm_l_m)
+
+pytestmark = pytest.mark.gpu
+
+CFG2 = ShellConfig(511, 257, 768, 1536)
+KS = np.array([0, 1, 128, 255, 256])
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _spectra_dev(t, cfg, ncomp, seed, slope=1.5):
+    l, _ = rlm_l_m(cfg.L)
+    r = cfg.radius()
+    gen = t.Generator(device="cuda")
+    gen.manual_seed(seed)
+    x = t.randn((cfg.nr, cfg.jmax, ncomp), generator=gen, dtype=t.float64, device="cuda")
+    x *= t.tensor((l + 1.0) ** -slope, device="cuda")[None, :, None]
+    prof = 0.5 + np.sin(np.pi * (r - cfg.ri) / (cfg.ro - cfg.ri))
+    x *= t.tensor(prof, device="cuda")[:, None, None]
+    return x.reshape(-1)
+
+
+def _rel(a, b, ncomp, comp_fastest):
+    a = np.asarray(a).reshape(-1, ncomp) if comp_fastest else np.asarray(a).reshape(ncomp, -1).T
+    b = np.asarray(b).reshape(-1, ncomp) if comp_fastest else np.asarray(b).reshape(ncomp, -1).T
+    return [float(np.linalg.norm(a[:, c] - b[:, c]) / np.linalg.norm(b[:, c]))
+            for c in range(ncomp)]
+
+
+@pytest.fixture(scope="module")
+def ref(torch_cuda):
+    """Inputs on the device and the oracle's transforms of them on KS."""
+    t = torch_cuda
+    cfg = CFG2
+    _say("[gpu] config 2 grid: inputs and oracle on radii", list(KS))
+    kd = t.from_numpy(KS).cuda()
+    sp_b = _spectra_dev(t, cfg, 8, 501)
+    plan = SphTransPlan(cfg, max_ncomp=9, precision=SHT_PREC_FP64)
+    try:
+        grid_f = t.empty(cfg.n_rtp(9), dtype=t.float64, device="cuda")
+        plan.sph_backward_transforms(3, 0, _spectra_dev(t, cfg, 9, 503), grid_f)
+        t.cuda.synchronize()
+    finally:
+        plan.close()
+    orc = O.OraclePlan(cfg.L, len(KS), cfg.nth, cfg.nphi, cfg.radius()[KS], full_tables=False)
+    sp_s = sp_b.view(cfg.nr, cfg.jmax, 8)[kd].contiguous().cpu().numpy().ravel()
+    g_s = grid_f.view(9, cfg.nphi, cfg.nth, cfg.nr)[:, :, :, kd].contiguous().cpu().numpy()
+    ref_v = orc.backward(2, 2, sp_s)
+    ref_s = orc.forward(3, 0, g_s.ravel())
+    _say("    oracle done")
+    yield sp_b, grid_f, ref_v, ref_s
+    del sp_b, grid_f
+    t.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("prec,tol", [(SHT_PREC_FP64, 1e-12), (SHT_PREC_SPLIT_INT8, 1e-11)],
+                         ids=["fp64", "split"])
+def test_whole_transforms_vs_oracle(torch_cuda, ref, prec, tol):
+    t = torch_cuda
+    cfg = CFG2
+    sp_b, grid_f, ref_v, ref_s = ref
+    kd = t.from_numpy(KS).cuda()
+    _say(f"[gpu] config 2 whole transforms, precision {prec}")
+    plan = SphTransPlan(cfg, max_ncomp=9, precision=prec)
+    try:
+        v = t.empty(cfg.n_rtp(8), dtype=t.float64, device="cuda")
+        s = t.empty(cfg.n_rj(9), dtype=t.float64, device="cuda")
+        plan.sph_backward_transforms(2, 2, sp_b, v)
+        plan.sph_forward_transforms(3, 0, grid_f, s)
+        t.cuda.synchronize()
+        got_v = v.view(8, cfg.nphi, cfg.nth, cfg.nr)[:, :, :, kd].contiguous().cpu().numpy()
+        got_s = s.view(cfg.nr, cfg.jmax, 9)[kd].contiguous().cpu().numpy()
+        del v, s
+    finally:
+        plan.close()
+        t.cuda.empty_cache()
+    eb = _rel(got_v, ref_v, 8, comp_fastest=False)
+    ef = _rel(got_s, ref_s, 9, comp_fastest=True)
+    _say("    backward rel L2 per component " + " ".join(f"{e:.2e}" for e in eb))
+    _say("    forward  rel L2 per component " + " ".join(f"{e:.2e}" for e in ef))
+    assert max(eb) <= tol and max(ef) <= tol
+
+
+def test_fft_stages_vs_oracle(torch_cuda):
+    """sht_fft_backward / sht_fft_forward on the reference's rtp buffers at
+    Nphi = 1536 (the split-form kernels with N' = 512, R0 = 3)."""
+    t = torch_cuda
+    cfg = CFG2
+    ncomp = 3
+    kd = t.from_numpy(KS).cuda()
+    _say("[gpu] config 2 FFT stages")
+    orc = O.OraclePlan(cfg.L, len(KS), cfg.nth, cfg.nphi, cfg.radius()[KS], full_tables=False)
+    plan = SphTransPlan(cfg, max_ncomp=ncomp)
+    gen = t.Generator(device="cuda")
+    gen.manual_seed(505)
+    try:
+        rtm = t.randn(cfg.n_rtm(ncomp), generator=gen, dtype=t.float64, device="cuda")
+        rtp = t.empty(cfg.n_rtp(ncomp), dtype=t.float64, device="cuda")
+        plan.exchange_rtm_to_rtp(ncomp, rtm, rtp)
+        v = t.empty(cfg.n_rtp(ncomp), dtype=t.float64, device="cuda")
+        plan.fft_backward(ncomp, rtp, v)
+        t.cuda.synchronize()
+        rtm_s = rtm.view(cfg.nslot, cfg.nr, cfg.nth, ncomp)[:, kd].contiguous().cpu().numpy()
+        ref = orc.fft_bwd_from_rtp(ncomp, orc.rtm_to_rtp(ncomp, rtm_s.ravel()))
+        got = v.view(ncomp, cfg.nphi, cfg.nth, cfg.nr)[:, :, :, kd].contiguous().cpu().numpy()
+        eb = _rel(got, ref, ncomp, comp_fastest=False)
+        del rtm, rtp
+        # forward of a grid that is not band-limited: compare the kept slots
+        g = t.randn(cfg.n_rtp(ncomp), generator=gen, dtype=t.float64, device="cuda")
+        out = v  # reuse the buffer
+        plan.fft_forward(ncomp, g, out)
+        t.cuda.synchronize()
+        g_s = g.view(ncomp, cfg.nphi, cfg.nth, cfg.nr)[:, :, :, kd].contiguous().cpu().numpy()
+        got_rtp = out.view(cfg.nphi, cfg.nth, cfg.nr, ncomp)[:, :, kd].contiguous().cpu().numpy()
+        ref_rtm = orc.rtp_to_rtm(ncomp, orc.fft_fwd_to_rtp(ncomp, g_s.ravel()))
+        got_rtm = orc.rtp_to_rtm(ncomp, got_rtp.ravel())
+        ef = _rel(got_rtm, ref_rtm, ncomp, comp_fastest=True)
+        del g, out, v
+    finally:
+        plan.close()
+        t.cuda.empty_cache()
+    _say(f"    FFT backward rel L2 {max(eb):.2e}, forward {max(ef):.2e}")
+    assert max(eb) <= 1e-12 and max(ef) <= 1e-12

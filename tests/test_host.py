@@ -135,13 +135,7 @@ def test_bench_refuses_more_than_one_gpu():
 AMDFLANG = "/opt/rocm/bin/amdflang"
 
 
-@pytest.mark.skipif(not os.path.exists(AMDFLANG), reason="no Fortran compiler")
-def test_fortran_binding_links_and_validates(tmp_path):
-    """The ISO_C_BINDING module compiles, links against the library and a
-    Fortran caller gets SHT_ERR_ARG for a bad descriptor (no device used)."""
-    mod = os.path.join(ROOT, "kemorin_mhd_amd", "fortran", "kemorin_sht_c.f90")
-    prog = tmp_path / "drv.f90"
-    prog.write_text("""
+_FORTRAN_DRIVER = """
       program drv
       use iso_c_binding
       use kemorin_sht_c
@@ -150,34 +144,100 @@ def test_fortran_binding_links_and_validates(tmp_path):
       type(sht_nonlinear_desc) :: nl
       type(c_ptr) :: plan
       real(c_double), target :: r(4)
-      integer(c_int) :: rc
+      integer(c_int) :: rc, icase
       r = (/0.5d0, 0.8d0, 1.1d0, 1.5d0/)
-      d%l_truncation = 0
-      d%nidx_r = 4
-      d%nidx_theta = 12
-      d%nidx_phi = 16
-      d%radius = c_loc(r)
-      d%max_ncomp = 9
-      d%precision = SHT_PREC_FP64
-      d%n_slices = 0
-      d%flags = 0
-      rc = sht_plan_create(d, plan)
-      write(*, '(a,i0)') 'rc=', rc
+      do icase = 0, 10
+        d%l_truncation = 3
+        d%nidx_r = 4
+        d%nidx_theta = 12
+        d%nidx_phi = 16
+        d%radius = c_loc(r)
+        d%max_ncomp = 9
+        d%precision = SHT_PREC_FP64
+        d%n_slices = 0
+        d%flags = 0
+        r(2) = 0.8d0
+        select case (icase)
+        case (1)
+          d%l_truncation = 0
+        case (2)
+          d%nidx_r = 0
+        case (3)
+          d%nidx_theta = 3
+        case (4)
+          d%nidx_phi = 15
+        case (5)
+          d%radius = c_null_ptr
+        case (6)
+          d%max_ncomp = 65
+        case (7)
+          d%precision = 7
+        case (8)
+          d%precision = SHT_PREC_SPLIT_INT8
+          d%n_slices = 3
+        case (9)
+          d%flags = 5
+        case (10)
+          r(2) = -1.0d0
+        end select
+        rc = sht_plan_create(d, plan)
+        write(*, '(a,i0,a,i0,2a)') 'case ', icase, ' rc=', rc, ' ', trim(last_error())
+      end do
       nl%i_velo = 0
       nl%i_coriolis = 0
       rc = sht_nonlinear_rtp(c_null_ptr, nl, 3_c_int, c_null_ptr,       &
      &                       3_c_int, c_null_ptr)
       write(*, '(a,i0)') 'nl=', rc
+      contains
+      function last_error() result(msg)
+      character(len = 256) :: msg
+      character(kind = c_char) :: buf(256)
+      integer(c_int) :: i, irc
+      buf = c_null_char
+      irc = sht_get_last_error(buf, 256_c_int)
+      msg = ' '
+      do i = 1, 256
+        if (buf(i) .eq. c_null_char) exit
+        msg(i:i) = buf(i)
+      end do
+      end function last_error
       end program drv
-""")
+"""
+
+# field -> (case number, text the message must carry: the field and its value)
+_FORTRAN_CASES = {
+    "l_truncation": (1, "l_truncation = 0"), "nidx_r": (2, "nidx_r = 0"),
+    "nidx_theta": (3, "nidx_theta = 3"), "nidx_phi": (4, "nidx_phi = 15"),
+    "radius": (5, "radius: null"), "max_ncomp": (6, "max_ncomp = 65"),
+    "precision": (7, "precision = 7"), "n_slices": (8, "n_slices = 3"),
+    "flags": (9, "flags = 5"), "radius values": (10, "radius[1] must be positive"),
+}
+
+
+@pytest.mark.skipif(not os.path.exists(AMDFLANG), reason="no Fortran compiler")
+def test_fortran_binding_lays_out_every_descriptor_field(tmp_path):
+    """The ISO_C_BINDING module compiles and links against the library.  A
+    Fortran caller breaks one sht_desc field at a time and gets SHT_ERR_ARG
+    with the message naming that field and the value it set, which pins
+    every field's offset in the bind(C) type; the unbroken descriptor passes
+    validation (and then finds no device here: SHT_ERR_HIP)."""
+    mod = os.path.join(ROOT, "kemorin_mhd_amd", "fortran", "kemorin_sht_c.f90")
+    prog = tmp_path / "drv.f90"
+    prog.write_text(_FORTRAN_DRIVER)
     exe = tmp_path / "drv"
     libdir = os.path.dirname(sht.LIB_PATH)
     r = subprocess.run([AMDFLANG, "-J", str(tmp_path), mod, str(prog), "-o", str(exe),
                         f"-L{libdir}", "-lkemorin_sht", f"-Wl,-rpath,{libdir}"],
                        capture_output=True, text=True, cwd=tmp_path)
     assert r.returncode == 0, r.stderr
-    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
-    assert "rc=1" in out.stdout, out.stdout + out.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120,
+                         env={**os.environ, "HIP_VISIBLE_DEVICES": ""})
+    lines = {int(ln.split()[1]): ln for ln in out.stdout.splitlines() if ln.startswith("case ")}
+    assert len(lines) == 11, out.stdout + out.stderr
+    assert " rc=4 " in lines[0] or lines[0].rstrip().endswith("rc=4"), lines[0]
+    for field, (case, text) in _FORTRAN_CASES.items():
+        assert " rc=1 " in lines[case], (field, lines[case])
+        assert text in lines[case], (field, lines[case])
     assert "nl=1" in out.stdout, out.stdout + out.stderr
 
 

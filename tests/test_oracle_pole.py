@@ -15,17 +15,21 @@ against what the mathematics fixes:
   of size 2c along z (order 0), x (cos 1) or y (sin 1), the same vector at
   both poles and at every radius.
 
-The vector centre row (SS/schmidt_b_trans_at_center.f90:66-115) follows the
-reference's code only: x takes the order -1 (sin) coefficient and y the
-order +1 (cos) one, both times -2 / r_c^2, which differs from the uniform-
-field value and from that file's own header comment; parity unpinned, the
-test below only checks which rows the centre writes.
+The centre row (SS/schmidt_b_trans_at_center.f90:45-115) is pinned against
+the reference's own routine, compiled from its source into oracle/_ref
+(oracle/Makefile.ref): bit for bit, vector and scalar components, with and
+without a prior content of the row.  (The reference's code takes x from the
+order -1 (sin) coefficient and y from order +1, both times -2 / r_c^2; its
+header comment names the cos mode for x.  The oracle follows the code.)
 """
 import numpy as np
 import pytest
 
 import oracle as O
+from oracle import ref
 from kemorin_mhd_amd import ShellConfig, gaussian, rlm_l_m
+
+needs_ref = pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built")
 
 
 def test_pole_tables_closed_forms():
@@ -121,3 +125,16 @@ def test_centre_row_writes_only_vector_components():
             j = int(np.nonzero((ls == 1) & (ms == m))[0][0])
             assert on[3 * f + comp, 2 * nr] == pytest.approx(sgn * 2.0 * s[j, 3 * f] * a * a,
                                                              rel=1e-15)
+
+
+@needs_ref
+@pytest.mark.parametrize("L,nr,nv,ns", [(5, 4, 2, 2), (31, 7, 1, 0), (12, 3, 0, 2), (3, 2, 3, 1)])
+def test_centre_row_bitexact_vs_reference(L, nr, nv, ns):
+    ncomp = 3 * nv + ns
+    c, orc = _plan(L, nr, 2 * L + 2)
+    R = ref.RefPlan(L, nr, c.nth, c.radius())
+    sp = gaussian(11 + L, c.n_rj(ncomp))
+    init = gaussian(12 + L, ncomp * (2 * nr + 1))
+    got = orc.pole_backward(nv, ns, sp, center=True, v_pole=init).reshape(ncomp, -1)[:, 2 * nr]
+    want = R.center(nv, ns, sp, init).reshape(ncomp, -1)[:, 2 * nr]
+    np.testing.assert_array_equal(got, want)
