@@ -101,6 +101,23 @@ int sht_backward_host(sht_plan *plan, int nvector, int nscalar,
 int sht_forward_host(sht_plan *plan, int nvector, int nscalar,
                      const double *v_rtp, double *WR_rj);
 
+/* No counterpart in the reference.  sht_backward for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  The call allocates nothing; its workspace
+ * (the Fourier workspace between the Legendre and FFT kernels and, for
+ * split-precision plans, the sliced data operand) is a fixed device array of
+ * the library, SHT_NOALLOC_SCRATCH_BYTES on each device, in place of the
+ * plan's.  Returns SHT_ERR_ARG when the call needs more than that (8 bytes x
+ * ncomp x (2L+1) x Nr x Ntheta, plus the data slices in split mode: e.g. up
+ * to L = 63, Nr = 65, Ntheta = 96 with 9 components in FP64, L = 31, Nr = 33,
+ * Ntheta = 48 in split mode) and SHT_ERR_STATE when the plan's stage timers
+ * are on.  Every call on a device shares that device's
+ * array and leaves its contents behind: calls that overlap on one device
+ * (other streams, plans or threads) race. */
+#define SHT_NOALLOC_SCRATCH_BYTES (128LL << 20)
+int sht_backward_noalloc(sht_plan *plan, int nvector, int nscalar,
+                         const double *d_WS_rj, double *d_v_rtp);
+
 /* Backward transform at the poles and the centre, device buffers.
  * Replaces pole_b_transform (SS/t_sph_transforms.f90:203-245), i.e.
  * pole_backward_transforms (SS/pole_sph_transform.f90:107-162) on one rank.

@@ -5,6 +5,7 @@
 // workspace already in rtp order, so the rj->rlm copy and the rtm->rtp
 // exchange of SS/t_sph_transforms.f90:74-123,250-297 cost no extra pass.
 #include <cstring>
+#include <mutex>
 
 #include "sht_internal.h"
 
@@ -111,6 +112,85 @@ extern "C" int sht_forward(sht_plan *p, int nv, int ns, const double *v_rtp, dou
   }
   if (rc) return rc;
   return leg_fwd(p, ws, nv, ns, ws.X, x_grid_addr(p), WR_rj, spec_addr(p, ncomp));
+}
+
+// ---------------------------------------------------------------------------
+// Fixed device workspace of sht_backward_noalloc.  Each device that loads the
+// library holds its own copy of this array until the process ends.  In host
+// code the array's name is only a stand-in for the symbol: its device address
+// comes from hipGetSymbolAddress, taken once per device with that device
+// current (scratch_attach, run at plan creation) and kept in scratch_dev.
+constexpr long long kScratchBytes = SHT_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex scratch_mu;
+static void *scratch_dev[kMaxDevices];
+
+namespace ksht {
+int scratch_attach(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("scratch: device %d outside 0..%d", device, kMaxDevices - 1);
+    return SHT_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_dev[device]) return SHT_OK;
+  int cur = -1;
+  KSHT_HIP(hipGetDevice(&cur));
+  if (cur != device) {
+    set_error("scratch: device %d is not current (current %d)", device, cur);
+    return SHT_ERR_STATE;
+  }
+  void *ptr = nullptr;
+  KSHT_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) {
+    set_error("scratch: no device address for the workspace array on device %d", device);
+    return SHT_ERR_HIP;
+  }
+  scratch_dev[device] = ptr;
+  return SHT_OK;
+}
+}  // namespace ksht
+
+static long long align256(long long n) { return (n + 255) / 256 * 256; }
+
+extern "C" int sht_backward_noalloc(sht_plan *p, int nv, int ns, const double *WS_rj,
+                                    double *v_rtp) {
+  int rc = check_call(p, nv, ns, WS_rj, v_rtp, "sht_backward_noalloc");
+  if (rc) return rc;
+  if (p->timing) {
+    set_error("sht_backward_noalloc: the plan's stage timers are on (they create events)");
+    return SHT_ERR_STATE;
+  }
+  const int ncomp = 3 * nv + ns;
+  const long long xb = align256((long long)sizeof(double) * ncomp * p->nslot * p->nr * p->nth);
+  long long db = 0, eb = 0;
+  if (p->precision == SHT_PREC_SPLIT_INT8) {
+    split_ws_bytes(p, ncomp, true, &db, &eb);
+    db = align256(db);
+    eb = align256(eb);
+  }
+  if (xb + db + eb > kScratchBytes) {
+    set_error("sht_backward_noalloc: the call needs %lld bytes of workspace, above the fixed "
+              "%lld (L=%d Nr=%d Ntheta=%d, %d components); use sht_backward",
+              xb + db + eb, kScratchBytes, p->L, p->nr, p->nth, ncomp);
+    return SHT_ERR_ARG;
+  }
+  void *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    base = (p->device >= 0 && p->device < kMaxDevices) ? scratch_dev[p->device] : nullptr;
+  }
+  if (!base) {
+    // plan creation could not attach it: this is the first call on the device
+    rc = scratch_attach(p->device);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    base = scratch_dev[p->device];
+  }
+  unsigned char *b = (unsigned char *)base;
+  const Workspace ws{(double *)b, db ? (signed char *)(b + xb) : nullptr,
+                     eb ? (int *)(b + xb + db) : nullptr};
+  return backward_ws(p, ws, nv, ns, WS_rj, v_rtp);
 }
 
 static int host_call(sht_plan *p, int nv, int ns, const double *in, double *out, bool bwd) {

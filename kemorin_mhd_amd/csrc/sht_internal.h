@@ -163,8 +163,9 @@ struct StageTimer {
 namespace ksht {
 // Per-call workspace of a whole transform: the field-major Fourier workspace
 // between the Legendre and FFT kernels and, for split-precision plans, the
-// sliced data operand (digits and exponents): the plan's own buffers
-// (plan_workspace).
+// sliced data operand (digits and exponents).  Normally the plan's own
+// buffers (plan_workspace); sht_backward_noalloc carves it from the library's
+// fixed device array instead.
 struct Workspace {
   double *X;
   signed char *dig;
@@ -175,6 +176,9 @@ Workspace plan_workspace(const sht_plan *p);
 // ncomp components in one direction
 void split_ws_bytes(const sht_plan *p, int ncomp, bool backward, long long *dig_bytes,
                     long long *dexp_bytes);
+// attaches the library's fixed device array on the current device (called at
+// plan creation; sht_capi.hip)
+int scratch_attach(int device);
 
 // Legendre launchers (sht_legendre.hip)
 int launch_leg_bwd(sht_plan *p, int nv, int ns, const double *sp, SpecAddr sa,

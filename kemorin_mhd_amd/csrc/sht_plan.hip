@@ -304,6 +304,9 @@ extern "C" int sht_plan_create(const sht_desc *d, sht_plan **out) {
     return SHT_ERR_HIP;
   }
   p->stream = p->own_stream;
+  // the fixed workspace of sht_backward_noalloc on this device; a failure
+  // here is reported by that entry point, which tries again
+  (void)scratch_attach(p->device);
   // host tables
   p->radius.assign(d->radius, d->radius + nr);
   p->gpoint.resize(nth);

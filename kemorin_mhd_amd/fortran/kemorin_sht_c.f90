@@ -103,6 +103,17 @@
           type(c_ptr), value :: d_WS_rj, d_v_rtp
         end function sht_backward
 !
+!>      No reference counterpart: sht_backward with the library's fixed
+!!      device workspace (for callers that cannot allocate)
+        integer(c_int) function sht_backward_noalloc(plan, nvector,    &
+     &                          nscalar, d_WS_rj, d_v_rtp)             &
+     &                          bind(C, name = 'sht_backward_noalloc')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: plan
+          integer(c_int), value :: nvector, nscalar
+          type(c_ptr), value :: d_WS_rj, d_v_rtp
+        end function sht_backward_noalloc
+!
 !>      sph_forward_transforms (SS/t_sph_transforms.f90:250-297), device
         integer(c_int) function sht_forward(plan, nvector, nscalar,    &
      &                          d_v_rtp, d_WR_rj)                      &

@@ -79,7 +79,7 @@ def lib() -> ctypes.CDLL:
         L.sht_plan_device_bytes.argtypes = [_vp]
         L.sht_plan_device_bytes.restype = ctypes.c_longlong
         for nm in ("sht_backward", "sht_forward", "sht_backward_host", "sht_forward_host",
-                   "sht_legendre_backward", "sht_legendre_forward"):
+                   "sht_legendre_backward", "sht_legendre_forward", "sht_backward_noalloc"):
             getattr(L, nm).argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]
         for nm in ("sht_fft_backward", "sht_fft_forward", "sht_exchange_rtm_to_rtp",
                    "sht_exchange_rtp_to_rtm"):
@@ -267,6 +267,14 @@ class SphTransPlan:
         self._bind(WS, v_rtp)
         _check(lib().sht_backward(self._h, nvector, nscalar, _ptr(WS), _ptr(v_rtp)),
                "sht_backward")
+
+    def sph_backward_transforms_noalloc(self, nvector: int, nscalar: int, WS, v_rtp):
+        """sph_backward_transforms with the library's fixed device workspace
+        (SHT_NOALLOC_SCRATCH_BYTES per device) instead of the plan's; the call
+        allocates nothing.  Calls that overlap on one device race."""
+        self._bind(WS, v_rtp)
+        _check(lib().sht_backward_noalloc(self._h, nvector, nscalar, _ptr(WS), _ptr(v_rtp)),
+               "sht_backward_noalloc")
 
     def sph_forward_transforms(self, nvector: int, nscalar: int, v_rtp, WR):
         """grid fields -> spectra (rj receive buffer), device buffers."""
