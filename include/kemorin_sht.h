@@ -159,6 +159,36 @@ int sht_nonlinear_rtp(sht_plan *plan, const sht_nonlinear_desc *desc,
                       int ncomp_fld, const double *d_fld_rtp,
                       int ncomp_frc, double *d_frc_rtp);
 
+/* Copies between the solver's field-major rj spectra d_fld[inod + nnod_rj*c]
+ * (c < ntot_phys) and the interleaved send / receive buffers of the
+ * transform.  A field of ncomp components (1 scalar, 3 vector, 6 tensor)
+ * sits at component i_field (0-based) of d_fld and i_trans of the buffer.
+ * All tables are device int arrays of 1-based node numbers, as the
+ * reference's comm_rj tables; NULL = the single-rank identity this engine
+ * uses (the buffer's node order is the rj order).  Up to 64 components per
+ * call; bit-exact copies, on the plan's stream. */
+typedef struct {
+  int i_field, i_trans, ncomp;
+} sht_field_copy;
+/* Replaces copy_mhd_spectr_to_send (MS/copy_sph_MHD_4_send_recv.f90:63-136)
+ * and its set_to_send_buf_* (SR/field_to_send_buffer.f90:43-120):
+ * WS[i_trans + c + ncomp_send*k] = d_fld[inod_export[k]-1 + nnod_rj*(i_field + c)]
+ * for k < nnod_send; components of WS no field names are left alone. */
+int sht_rj_fields_to_send(sht_plan *plan, int nfield, const sht_field_copy *fields,
+                          int ntot_phys, long long nnod_rj, const double *d_fld,
+                          const int *d_inod_export, long long nnod_send, int ncomp_send,
+                          double *d_WS);
+/* Replaces copy_mhd_spectr_from_recv (MS/copy_sph_MHD_4_send_recv.f90:140-206)
+ * with the reverse-table copies set_from_recv_buf_rev_*
+ * (SR/field_to_send_buffer.f90:246-330):
+ * d_fld[k + nnod_rj*(i_field + c)] = WR[i_trans + c + ncomp_recv*(irev[k]-1)]
+ * for k < nnod_rj, zero where irev[k] = nnod_recv + 1 (the dummy record; the
+ * reference zeroes it inside WR, here WR is not written). */
+int sht_rj_fields_from_recv(sht_plan *plan, int nfield, const sht_field_copy *fields,
+                            int ncomp_recv, long long nnod_recv, const double *d_WR,
+                            const int *d_irev, int ntot_phys, long long nnod_rj,
+                            double *d_fld);
+
 /* Stage entry points on the reference's buffers, for callers that run work
  * between stages (the Coriolis hooks of MS/sph_trans_w_coriols.f90:146,233).
  * Legendre: replaces sel_backward_legendre_trans / sel_forward_legendre_trans

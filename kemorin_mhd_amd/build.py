@@ -15,7 +15,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkemorin_sht.so")
 SOURCES = ["sht_plan.hip", "sht_legendre.hip", "sht_fft.hip", "sht_reorder.hip",
            "sht_split.hip", "sht_capi.hip", "sht_batch.hip", "sht_pole.hip",
-           "sht_nonlinear.hip"]
+           "sht_nonlinear.hip", "sht_copy.hip"]
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-fno-gpu-rdc",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -29,12 +29,30 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> str:
+HEADERS = [os.path.join(CSRC, "sht_internal.h"),
+           os.path.join(os.path.dirname(HERE), "include", "kemorin_sht.h")]
+
+
+def _stale(src: str, obj: str) -> bool:
+    """An object older than its source or a shared header."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(f) > t for f in [os.path.join(CSRC, src), *HEADERS])
+
+
+def build(verbose: bool = False, incremental: bool = False) -> str:
+    """Compile every source (incremental: only those whose object is older
+    than the source or a shared header) and link the library."""
     os.makedirs(OBJ, exist_ok=True)
     cc = hipcc()
+    todo = [s for s in SOURCES
+            if not incremental or _stale(s, os.path.join(OBJ, s.replace(".hip", ".o")))]
 
     def one(src):
         obj = os.path.join(OBJ, src.replace(".hip", ".o"))
+        if src not in todo:
+            return obj
         cmd = [cc, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
@@ -50,6 +68,8 @@ def build(verbose: bool = False) -> str:
         # the wait states the assembly needs, on all paths
         from .isa_waits import check_source
         for src, pattern in (("sht_split.hip", "k_leg_"), ("sht_fft.hip", "k_fft3_")):
+            if src not in todo:
+                continue
             probs = check_source(cc, FLAGS, os.path.join(CSRC, src), pattern)
             bad = {k: v for k, v in probs.items() if v}
             if not probs or bad:
@@ -70,4 +90,4 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))

@@ -169,8 +169,11 @@ __global__ void __launch_bounds__(256) k_leg_bwd(
               acc[sg][par][3] = mfma64(adP, bdpol, acc[sg][par][3]);
               acc[sg][par][4] = mfma64(adP, btor, acc[sg][par][4]);
             } else {
+              // nc is uniform over the work-group: a group of fewer than
+              // kSclPer scalars skips the products of its empty components
 #pragma unroll
-              for (int c = 0; c < NQ; c++) acc[sg][par][c] = mfma64(aP, bb[c * 512], acc[sg][par][c]);
+              for (int c = 0; c < NQ; c++)
+                if (c < nc) acc[sg][par][c] = mfma64(aP, bb[c * 512], acc[sg][par][c]);
             }
           }
         }
@@ -385,7 +388,7 @@ __global__ void __launch_bounds__(256) k_leg_fwd(
             } else {
 #pragma unroll
               for (int c = 0; c < NQ; c++)
-                acc[sg][par][c] = mfma64(aa[(c * 2 + sP) * 256], bP, acc[sg][par][c]);
+                if (c < nc) acc[sg][par][c] = mfma64(aa[(c * 2 + sP) * 256], bP, acc[sg][par][c]);
             }
           }
         }

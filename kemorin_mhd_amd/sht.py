@@ -59,6 +59,10 @@ NL_OUTPUTS = ("m_advect", "lorentz", "vp_induct", "h_flux", "c_flux", "coriolis"
 _NL_COEF = dict(zip(NL_OUTPUTS, ("velo", "lor", "induct", "h_advect", "c_advect", "cor")))
 
 
+class _FieldCopy(ctypes.Structure):
+    _fields_ = [("i_field", ctypes.c_int), ("i_trans", ctypes.c_int), ("ncomp", ctypes.c_int)]
+
+
 class _ShellCall(ctypes.Structure):
     _fields_ = [("plan", _vp), ("nvector_b", ctypes.c_int), ("nscalar_b", ctypes.c_int),
                 ("d_WS_rj", _vp), ("d_v_rtp", _vp), ("nvector_f", ctypes.c_int),
@@ -91,6 +95,11 @@ def lib() -> ctypes.CDLL:
                                         ctypes.c_int, _vp]
         L.sht_exchange_gather_rev.argtypes = [_vp, ctypes.c_int, ctypes.c_longlong,
                                               ctypes.c_longlong, _vp, _vp, _vp]
+        _ll = ctypes.c_longlong
+        L.sht_rj_fields_to_send.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_FieldCopy),
+                                            ctypes.c_int, _ll, _vp, _vp, _ll, ctypes.c_int, _vp]
+        L.sht_rj_fields_from_recv.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_FieldCopy),
+                                              ctypes.c_int, _ll, _vp, _vp, ctypes.c_int, _ll, _vp]
         L.sht_plan_timer_enable.argtypes = [_vp, ctypes.c_int]
         L.sht_plan_timer_read.argtypes = [_vp, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
@@ -315,6 +324,41 @@ class SphTransPlan:
         self._bind(fld_rtp, frc_rtp)
         _check(lib().sht_nonlinear_rtp(self._h, ctypes.byref(d), ncomp_fld, _ptr(fld_rtp),
                                        ncomp_frc, _ptr(frc_rtp)), "sht_nonlinear_rtp")
+
+    @staticmethod
+    def _field_list(fields):
+        arr = (_FieldCopy * max(len(fields), 1))()
+        for i, (i_field, i_trans, nc) in enumerate(fields):
+            arr[i] = _FieldCopy(int(i_field), int(i_trans), int(nc))
+        return arr
+
+    def copy_spectr_to_send(self, fields, ntot_phys: int, d_fld, ncomp_send: int, WS,
+                            inod_export=None):
+        """Field-major rj spectra d_fld[inod + nnod_rj c] -> interleaved send
+        buffer WS (copy_mhd_spectr_to_send, MS/copy_sph_MHD_4_send_recv.f90:
+        63-136).  fields: (i_field, i_trans, ncomp) triples, 0-based;
+        inod_export: device int32 tensor of 1-based nodes or None (identity)."""
+        nnod_rj = d_fld.numel() // ntot_phys
+        nnod_send = WS.numel() // ncomp_send
+        self._bind(d_fld, WS)
+        _check(lib().sht_rj_fields_to_send(
+            self._h, len(fields), self._field_list(fields), ntot_phys, nnod_rj, _ptr(d_fld),
+            inod_export.data_ptr() if inod_export is not None else None, nnod_send, ncomp_send,
+            _ptr(WS)), "sht_rj_fields_to_send")
+
+    def copy_spectr_from_recv(self, fields, ncomp_recv: int, WR, ntot_phys: int, d_fld,
+                              irev=None):
+        """Interleaved receive buffer WR -> field-major rj spectra
+        (copy_mhd_spectr_from_recv, MS/copy_sph_MHD_4_send_recv.f90:140-206);
+        irev: device int32 tensor of 1-based records (nnod_recv + 1 = zero)
+        or None (identity)."""
+        nnod_rj = d_fld.numel() // ntot_phys
+        nnod_recv = WR.numel() // ncomp_recv
+        self._bind(WR, d_fld)
+        _check(lib().sht_rj_fields_from_recv(
+            self._h, len(fields), self._field_list(fields), ncomp_recv, nnod_recv, _ptr(WR),
+            irev.data_ptr() if irev is not None else None, ntot_phys, nnod_rj, _ptr(d_fld)),
+            "sht_rj_fields_from_recv")
 
     def backward_host(self, nvector, nscalar, WS: np.ndarray) -> np.ndarray:
         ncomp = 3 * nvector + nscalar

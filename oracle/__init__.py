@@ -61,6 +61,11 @@ def lib() -> ctypes.CDLL:
                                         ctypes.c_int]
         L.orc_nonlinear_rtp.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), _dp,
                                         ctypes.c_int, _dp, ctypes.c_int, _dp]
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_rj_to_send.argtypes = [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_longlong, _dp,
+                                     _ip, ctypes.c_longlong, ctypes.c_int, _dp]
+        L.orc_rj_from_recv.argtypes = [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_longlong, _dp,
+                                       _ip, ctypes.c_int, ctypes.c_longlong, _dp]
         L.orc_num_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -69,6 +74,42 @@ def lib() -> ctypes.CDLL:
 def _ptr(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(_dp)
+
+
+def _fields_arr(fields):
+    a = np.ascontiguousarray(np.asarray(fields, dtype=np.int32).reshape(-1, 3))
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _table(t):
+    if t is None:
+        return None, None
+    a = np.ascontiguousarray(np.asarray(t, dtype=np.int32))
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def rj_to_send(fields, ntot_phys, nnod_rj, fld, nb, WS, inod_export=None, nnod_send=None):
+    """fields: (i_field, i_trans, ncomp) triples, 0-based; WS is updated in
+    place (components no field names are kept) and returned."""
+    fa, fp = _fields_arr(fields)
+    ta, tp = _table(inod_export)
+    fld = np.ascontiguousarray(fld, dtype=np.float64)
+    WS = np.array(WS, dtype=np.float64)
+    n_send = nnod_rj if nnod_send is None else nnod_send
+    lib().orc_rj_to_send(len(fa), fp, ntot_phys, nnod_rj, _ptr(fld), tp, n_send, nb, _ptr(WS))
+    return WS
+
+
+def rj_from_recv(fields, nb, nnod_recv, WR, ntot_phys, nnod_rj, fld, irev=None):
+    """fld (field-major, ntot_phys x nnod_rj) is updated in place for the
+    named components and returned."""
+    fa, fp = _fields_arr(fields)
+    ta, tp = _table(irev)
+    WR = np.ascontiguousarray(WR, dtype=np.float64)
+    fld = np.array(fld, dtype=np.float64)
+    lib().orc_rj_from_recv(len(fa), fp, nb, nnod_recv, _ptr(WR), tp, ntot_phys, nnod_rj,
+                           _ptr(fld))
+    return fld
 
 
 def gauss_points(n: int, exact: bool = False):

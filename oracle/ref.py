@@ -42,6 +42,9 @@ def lib():
         L.ref_leg_bwd.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.ref_leg_fwd.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.ref_center.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.ref_to_send_buf.argtypes = [ctypes.c_int] * 4 + [_ip] + [ctypes.c_int] * 3 + [_dp, _dp]
+        L.ref_from_recv_rev.argtypes = [ctypes.c_int] * 4 + [_ip] + [ctypes.c_int] * 3 + [_dp, _dp]
         _LIB = L
     return _LIB
 
@@ -87,6 +90,33 @@ def rfft_bwd(c):
     buf = np.ascontiguousarray(lines.T)
     lib().ref_rfftm(-1, n, lines.shape[0], _p(buf))
     return np.ascontiguousarray(buf.T).reshape(c.shape)
+
+
+def to_send(fields, ntot_phys, nnod_org, d_org, nb, WS, inod_export):
+    """The reference's set_to_send_buf_* for each (i_field, i_trans, ncomp)
+    (0-based) in turn; d_org field-major (ntot_phys x nnod_org)."""
+    d = np.ascontiguousarray(d_org, dtype=np.float64).ravel()
+    ws = np.array(WS, dtype=np.float64).ravel()
+    inod = np.ascontiguousarray(inod_export, dtype=np.int32)
+    nsend = ws.size // nb
+    for i_fld, i_tr, nc in fields:
+        lib().ref_to_send_buf(nc, nb, nnod_org, nsend, inod.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                              ntot_phys, i_fld + 1, i_tr + 1, _p(d), _p(ws))
+    return ws
+
+
+def from_recv_rev(fields, nb, nnod_recv, WR, ntot_phys, nnod_new, d_new, irev):
+    """The reference's set_from_recv_buf_rev_* for each field in turn; WR is
+    extended by the dummy record the routines zero."""
+    wr = np.zeros(nb * (nnod_recv + 1))
+    wr[:nb * nnod_recv] = np.asarray(WR, dtype=np.float64).ravel()[:nb * nnod_recv]
+    d = np.array(d_new, dtype=np.float64).ravel()
+    rv = np.ascontiguousarray(irev, dtype=np.int32)
+    for i_fld, i_tr, nc in fields:
+        lib().ref_from_recv_rev(nc, nb, nnod_new, nnod_recv,
+                                rv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ntot_phys,
+                                i_fld + 1, i_tr + 1, _p(wr), _p(d))
+    return d
 
 
 class RefPlan:

@@ -27,6 +27,7 @@
       use set_vr_rtm_for_leg_matmul
       use cal_sp_rlm_by_matmul
       use schmidt_b_trans_at_center
+      use field_to_send_buffer
 !
       implicit none
 !
@@ -226,6 +227,76 @@
      &    v_pole)
       deallocate(irev)
       end subroutine ref_center
+!
+!> One field to the interleaved send buffer with the reference's
+!! set_to_send_buf_scalar / _vector / _tensor (SR/field_to_send_buffer.f90);
+!! i_fld_X, i_fld_WS and inod_export 1-based as in the reference.
+      subroutine ref_to_send_buf(ncomp, NB, nnod_org, nnod_send,         &
+     &          inod_export, ncomp_X, i_fld_X, i_fld_WS, d_org, WS)      &
+     &          bind(C, name='ref_to_send_buf')
+      integer(c_int), value :: ncomp, NB, nnod_org, nnod_send
+      integer(c_int), value :: ncomp_X, i_fld_X, i_fld_WS
+      integer(c_int), intent(in) :: inod_export(nnod_send)
+      real(c_double), intent(in) :: d_org(nnod_org, ncomp_X)
+      real(c_double), intent(inout) :: WS(NB*nnod_send)
+      integer(kind = kint) :: nbk, nok, nsk, nxk, ifx, ifw
+      integer(kind = kint), allocatable :: inod(:)
+      nbk = NB
+      nok = nnod_org
+      nsk = nnod_send
+      nxk = ncomp_X
+      ifx = i_fld_X
+      ifw = i_fld_WS
+      allocate(inod(nsk))
+      inod(1:nsk) = inod_export(1:nsk)
+      select case(ncomp)
+      case(1)
+        call set_to_send_buf_scalar(nbk, nok, nsk, inod, nxk, ifx,      &
+     &      ifw, d_org, WS)
+      case(3)
+        call set_to_send_buf_vector(nbk, nok, nsk, inod, nxk, ifx,      &
+     &      ifw, d_org, WS)
+      case(6)
+        call set_to_send_buf_tensor(nbk, nok, nsk, inod, nxk, ifx,      &
+     &      ifw, d_org, WS)
+      end select
+      deallocate(inod)
+      end subroutine ref_to_send_buf
+!
+!> One field from the receive buffer by the reverse table with the
+!! reference's set_from_recv_buf_rev_scalar / _vector / _tensor; WR holds
+!! NB*(nnod_recv+1) values (the routine zeroes the dummy record).
+      subroutine ref_from_recv_rev(ncomp, NB, nnod_new, nnod_recv,      &
+     &          irev_import, ncomp_X, i_fld_X, i_fld_WR, WR, d_new)     &
+     &          bind(C, name='ref_from_recv_rev')
+      integer(c_int), value :: ncomp, NB, nnod_new, nnod_recv
+      integer(c_int), value :: ncomp_X, i_fld_X, i_fld_WR
+      integer(c_int), intent(in) :: irev_import(nnod_new)
+      real(c_double), intent(inout) :: WR(NB*(nnod_recv+1))
+      real(c_double), intent(inout) :: d_new(nnod_new, ncomp_X)
+      integer(kind = kint) :: nbk, nnk, nrk, nxk, ifx, ifw
+      integer(kind = kint), allocatable :: irev(:)
+      nbk = NB
+      nnk = nnod_new
+      nrk = nnod_recv
+      nxk = ncomp_X
+      ifx = i_fld_X
+      ifw = i_fld_WR
+      allocate(irev(nnk))
+      irev(1:nnk) = irev_import(1:nnk)
+      select case(ncomp)
+      case(1)
+        call set_from_recv_buf_rev_scalar(nbk, nnk, nrk, irev, nxk,     &
+     &      ifx, ifw, WR, d_new)
+      case(3)
+        call set_from_recv_buf_rev_vector(nbk, nnk, nrk, irev, nxk,     &
+     &      ifx, ifw, WR, d_new)
+      case(6)
+        call set_from_recv_buf_rev_tensor(nbk, nnk, nrk, irev, nxk,     &
+     &      ifx, ifw, WR, d_new)
+      end select
+      deallocate(irev)
+      end subroutine ref_from_recv_rev
 !
       subroutine ref_release() bind(C, name='ref_release')
       if(allocated(radius_h)) deallocate(radius_h, a_r_h)

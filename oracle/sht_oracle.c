@@ -1279,3 +1279,41 @@ void orc_nonlinear_rtp(const orc_plan *p, const int idx[12],
     }
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* rj fields <-> interleaved send / receive buffers,
+ * SR/field_to_send_buffer.f90:43-120 (to send) and :246-330 (from the
+ * receive buffer by the reverse table); per field the reference routine of
+ * its component count, in the order of the field list. */
+void orc_rj_to_send(int nfield, const int *fields, int ntot_phys, long long nnod_rj,
+                    const double *fld, const int *inod_export, long long nnod_send, int nb,
+                    double *WS) {
+  int f, c;
+  long long k;
+  (void)ntot_phys;
+  for (f = 0; f < nfield; f++) {
+    const int i_fld = fields[3 * f], i_ws = fields[3 * f + 1], nc = fields[3 * f + 2];
+    for (k = 0; k < nnod_send; k++) {
+      const long long kk = (long long)nb * k + i_ws;
+      const long long jj = inod_export ? (long long)inod_export[k] - 1 : k;
+      for (c = 0; c < nc; c++) WS[kk + c] = fld[jj + nnod_rj * (long long)(i_fld + c)];
+    }
+  }
+}
+
+void orc_rj_from_recv(int nfield, const int *fields, int nb, long long nnod_recv,
+                      const double *WR, const int *irev, int ntot_phys, long long nnod_rj,
+                      double *fld) {
+  int f, c;
+  long long k;
+  (void)ntot_phys;
+  for (f = 0; f < nfield; f++) {
+    const int i_fld = fields[3 * f], i_wr = fields[3 * f + 1], nc = fields[3 * f + 2];
+    for (k = 0; k < nnod_rj; k++) {
+      const long long r = irev ? (long long)irev[k] - 1 : k;
+      for (c = 0; c < nc; c++)
+        fld[k + nnod_rj * (long long)(i_fld + c)] =
+            r == nnod_recv ? 0.0 : WR[(long long)nb * r + i_wr + c];
+    }
+  }
+}

@@ -128,6 +128,23 @@ void orc_nonlinear_rtp(const orc_plan *p, const int idx[12],
                        const double coef[6], int ncomp_fld, const double *fld,
                        int ncomp_frc, double *frc);
 
+/* Copies between field-major rj spectra fld[inod + nnod_rj*c] and the
+ * interleaved send / receive buffers, per field (fields[3f..3f+2] =
+ * i_field, i_trans (0-based first components), ncomp 1, 3 or 6).
+ * to_send: SR/field_to_send_buffer.f90:43-120 (set_to_send_buf_*),
+ *   WS[i_trans + c + nb*k] = fld[inod_export[k]-1 + nnod_rj*(i_field + c)];
+ * from_recv: :246-330 (set_from_recv_buf_rev_*), the dummy record nnod_recv+1
+ *   reading zero:
+ *   fld[k + nnod_rj*(i_field + c)] = WR[i_trans + c + nb*(irev[k]-1)].
+ * Tables are 1-based; NULL = identity.  Called by
+ * MS/copy_sph_MHD_4_send_recv.f90:63-206. */
+void orc_rj_to_send(int nfield, const int *fields, int ntot_phys, long long nnod_rj,
+                    const double *fld, const int *inod_export, long long nnod_send, int nb,
+                    double *WS);
+void orc_rj_from_recv(int nfield, const int *fields, int nb, long long nnod_recv,
+                      const double *WR, const int *irev, int ntot_phys, long long nnod_rj,
+                      double *fld);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus
