@@ -189,6 +189,45 @@ int sht_rj_fields_from_recv(sht_plan *plan, int nfield, const sht_field_copy *fi
                             const int *d_irev, int ntot_phys, long long nnod_rj,
                             double *d_fld);
 
+/* Coriolis term in rlm space, for rotating runs whose Coriolis force is
+ * taken by Gaunt integrals between the stages of the MHD step
+ * (MS/sph_trans_w_coriols.f90:146-149, 233-235;
+ * MS/ = MHD/Fortran_src/MHD_src/sph_MHD/).  All buffers are device buffers
+ * on the plan's single-rank rlm layout; calls run on the plan's stream.
+ * Replaces init_sum_coriolis_rlm (MS/t_coriolis_terms_rlm.f90:133-183):
+ * the Gaunt and Elsasser integrals of cal_gaunt_coriolis_rlm
+ * (MS/interact_coriolis_rlm.f90:42-118), the coefficients of
+ * interact_rot_coriolis_rlm (:122-205) and the rotation spectrum of
+ * set_zdir_rot_earth_4_sph (MS/t_poloidal_rotation.f90:155-250: r^2/2, r,
+ * 1).  kr_icb: 1-based radius of a rotatable inner core boundary
+ * (sph_bc_U%iflag_icb = iflag_rotatable_ic), 0 = none. */
+typedef struct sht_coriolis sht_coriolis;
+int sht_coriolis_create(sht_plan *plan, int kr_icb, sht_coriolis **cor);
+int sht_coriolis_destroy(sht_coriolis *cor);
+/* Replaces sum_coriolis_rlm (MS/t_coriolis_terms_rlm.f90:187-258), i.e.
+ * sum_rot_coriolis_rlm_10 (MS/sum_coriolis_terms_rlm.f90:96-186) and, with
+ * kr_icb, inner_core_rot_z_coriolis_rlm (:378-418): the curl of
+ * -coef_cor (z x u), poloidal part to d_cor_rlm[i] and toroidal part to
+ * d_cor_rlm[i + nnod_rlm], i = j + jmax*k.  d_WR: receive buffer of
+ * ncomp_trans interleaved components with velocity and vorticity
+ * (pol, dpol/dr, tor) at 0-based components i_velo and i_vort; record of
+ * node i = d_irev_sr_rlm[i] - 1 (1-based device int table; NULL = the
+ * single-rank identity); records at or past nnod_recv (the dummy record)
+ * read zero. */
+int sht_sum_coriolis_rlm(sht_coriolis *cor, int ncomp_trans, int i_velo, int i_vort,
+                         double coef_cor, const double *d_WR, long long nnod_recv,
+                         const int *d_irev_sr_rlm, double *d_cor_rlm);
+/* Replaces copy_coriolis_terms_rlm (MS/t_coriolis_terms_rlm.f90:262-296):
+ * the poloidal and toroidal parts of d_cor_rlm go to components
+ * i_rot_coriolis and i_rot_coriolis + 2 (0-based) of the forward transform's
+ * send buffer, WS[c + ncomp_trans*k] = d_cor_rlm[item[k]-1 (+ nnod_rlm)]
+ * for k < nnod_send (item: 1-based device int table of the rlm send
+ * order, NULL = identity with nnod_send = nnod_rlm).  Other components of
+ * WS are left alone.  Bit-exact copies. */
+int sht_copy_coriolis_terms_rlm(sht_coriolis *cor, int ncomp_trans, int i_rot_coriolis,
+                                const double *d_cor_rlm, const int *d_item_sr_rlm,
+                                long long nnod_send, double *d_WS);
+
 /* Stage entry points on the reference's buffers, for callers that run work
  * between stages (the Coriolis hooks of MS/sph_trans_w_coriols.f90:146,233).
  * Legendre: replaces sel_backward_legendre_trans / sel_forward_legendre_trans

@@ -107,6 +107,32 @@ static int make_map(const char *who, int nfield, const sht_field_copy *f, int nt
   return SHT_OK;
 }
 
+// Named component pairs (fld[i] -> trn[i]) of a field-major array to an
+// interleaved send buffer, for callers that copy single components
+// (copy_coriolis_terms_rlm, sht_coriolis.hip).
+int launch_to_send_pairs(sht_plan *p, int n, const int *fld, const int *trn, long long nnod_fld,
+                         const double *d_fld, const int *d_inod, long long nnod_send, int nb,
+                         double *d_WS) {
+  if (n < 0 || n > kMaxCopyComps) {
+    set_error("to_send: %d components (1..%d)", n, kMaxCopyComps);
+    return SHT_ERR_ARG;
+  }
+  CopyMap cm;
+  cm.n = n;
+  for (int i = 0; i < n; i++) {
+    cm.fld[i] = fld[i];
+    cm.trn[i] = trn[i];
+  }
+  if (n == 0 || nnod_send == 0) return SHT_OK;
+  StageTimer tm(p, 4);
+  const long long nblk = (nnod_send + kCopyNodes - 1) / kCopyNodes;
+  hipLaunchKernelGGL(k_to_send, dim3((unsigned)nblk), dim3(kCopyThreads),
+                     sizeof(double) * kCopyNodes * cm.n, p->stream, cm, nnod_fld, d_fld, d_inod,
+                     nnod_send, nb, d_WS);
+  KSHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+
 }  // namespace ksht
 
 using namespace ksht;

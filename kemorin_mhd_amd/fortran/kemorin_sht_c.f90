@@ -249,6 +249,73 @@
           type(c_ptr), value :: d_irev, d_src, d_dst
         end function sht_exchange_gather_rev
 !
+!       rj fields <-> interleaved send / receive buffers (sht_copy.hip)
+        integer(c_int) function sht_rj_fields_to_send(plan, nfield,    &
+     &          fields, ntot_phys, nnod_rj, d_fld, d_inod_export,       &
+     &          nnod_send, ncomp_send, d_WS)                            &
+     &                          bind(C, name = 'sht_rj_fields_to_send')
+          import :: c_int, c_ptr, c_long_long
+          type(c_ptr), value :: plan
+          integer(c_int), value :: nfield
+          integer(c_int), intent(in) :: fields(3, *)
+          integer(c_int), value :: ntot_phys, ncomp_send
+          integer(c_long_long), value :: nnod_rj, nnod_send
+          type(c_ptr), value :: d_fld, d_inod_export, d_WS
+        end function sht_rj_fields_to_send
+!
+        integer(c_int) function sht_rj_fields_from_recv(plan, nfield,  &
+     &          fields, ncomp_recv, nnod_recv, d_WR, d_irev, ntot_phys, &
+     &          nnod_rj, d_fld)                                         &
+     &                          bind(C, name = 'sht_rj_fields_from_recv')
+          import :: c_int, c_ptr, c_long_long
+          type(c_ptr), value :: plan
+          integer(c_int), value :: nfield
+          integer(c_int), intent(in) :: fields(3, *)
+          integer(c_int), value :: ncomp_recv, ntot_phys
+          integer(c_long_long), value :: nnod_recv, nnod_rj
+          type(c_ptr), value :: d_WR, d_irev, d_fld
+        end function sht_rj_fields_from_recv
+!
+!       Coriolis term in rlm space (sht_coriolis.hip)
+        integer(c_int) function sht_coriolis_create(plan, kr_icb, cor) &
+     &                          bind(C, name = 'sht_coriolis_create')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: plan
+          integer(c_int), value :: kr_icb
+          type(c_ptr), intent(out) :: cor
+        end function sht_coriolis_create
+!
+        integer(c_int) function sht_coriolis_destroy(cor)              &
+     &                          bind(C, name = 'sht_coriolis_destroy')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: cor
+        end function sht_coriolis_destroy
+!
+        integer(c_int) function sht_sum_coriolis_rlm(cor, ncomp_trans, &
+     &          i_velo, i_vort, coef_cor, d_WR, nnod_recv,              &
+     &          d_irev_sr_rlm, d_cor_rlm)                               &
+     &                          bind(C, name = 'sht_sum_coriolis_rlm')
+          import :: c_int, c_ptr, c_double, c_long_long
+          type(c_ptr), value :: cor
+          integer(c_int), value :: ncomp_trans, i_velo, i_vort
+          real(c_double), value :: coef_cor
+          type(c_ptr), value :: d_WR
+          integer(c_long_long), value :: nnod_recv
+          type(c_ptr), value :: d_irev_sr_rlm, d_cor_rlm
+        end function sht_sum_coriolis_rlm
+!
+        integer(c_int) function sht_copy_coriolis_terms_rlm(cor,       &
+     &          ncomp_trans, i_rot_coriolis, d_cor_rlm, d_item_sr_rlm,  &
+     &          nnod_send, d_WS)                                        &
+     &                  bind(C, name = 'sht_copy_coriolis_terms_rlm')
+          import :: c_int, c_ptr, c_long_long
+          type(c_ptr), value :: cor
+          integer(c_int), value :: ncomp_trans, i_rot_coriolis
+          type(c_ptr), value :: d_cor_rlm, d_item_sr_rlm
+          integer(c_long_long), value :: nnod_send
+          type(c_ptr), value :: d_WS
+        end function sht_copy_coriolis_terms_rlm
+!
         integer(c_int) function sht_selftest()                         &
      &                          bind(C, name = 'sht_selftest')
           import :: c_int

@@ -100,6 +100,12 @@ def lib() -> ctypes.CDLL:
                                             ctypes.c_int, _ll, _vp, _vp, _ll, ctypes.c_int, _vp]
         L.sht_rj_fields_from_recv.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_FieldCopy),
                                               ctypes.c_int, _ll, _vp, _vp, ctypes.c_int, _ll, _vp]
+        L.sht_coriolis_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.sht_coriolis_destroy.argtypes = [_vp]
+        L.sht_sum_coriolis_rlm.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_double, _vp, _ll, _vp, _vp]
+        L.sht_copy_coriolis_terms_rlm.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _ll,
+                                                  _vp]
         L.sht_plan_timer_enable.argtypes = [_vp, ctypes.c_int]
         L.sht_plan_timer_read.argtypes = [_vp, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
@@ -360,6 +366,10 @@ class SphTransPlan:
             irev.data_ptr() if irev is not None else None, ntot_phys, nnod_rj, _ptr(d_fld)),
             "sht_rj_fields_from_recv")
 
+    def coriolis(self, kr_icb: int = 0) -> "CoriolisRlm":
+        """Coriolis term in rlm space on this plan (init_sum_coriolis_rlm)."""
+        return CoriolisRlm(self, kr_icb)
+
     def backward_host(self, nvector, nscalar, WS: np.ndarray) -> np.ndarray:
         ncomp = 3 * nvector + nscalar
         out = np.empty(self.cfg.n_rtp(ncomp))
@@ -410,6 +420,52 @@ class SphTransPlan:
         self._bind(src, dst)
         _check(lib().sht_exchange_gather_rev(self._h, ncomp, n_dst, n_src, irev.data_ptr(),
                                              _ptr(src), _ptr(dst)), "sht_exchange_gather_rev")
+
+
+class CoriolisRlm:
+    """Coriolis term in rlm space (MS/t_coriolis_terms_rlm.f90): tables of
+    init_sum_coriolis_rlm at creation, then sum_coriolis_rlm after the
+    rj -> rlm exchange and copy_coriolis_terms_rlm after the forward
+    Legendre transform (MS/sph_trans_w_coriols.f90:146-149, 233-235)."""
+
+    def __init__(self, plan: "SphTransPlan", kr_icb: int = 0):
+        self.plan = plan
+        h = _vp()
+        _check(lib().sht_coriolis_create(plan.handle, kr_icb, ctypes.byref(h)),
+               "sht_coriolis_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sht_coriolis_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sum_coriolis_rlm(self, ncomp_trans: int, i_velo: int, i_vort: int, coef_cor: float, WR,
+                         d_cor, irev=None):
+        """Curl of -coef_cor (z x u): d_cor[i] poloidal, d_cor[i + nnod_rlm]
+        toroidal.  i_velo / i_vort: 0-based first components in WR;
+        irev: device int32 tensor of 1-based records or None (identity)."""
+        nnod_recv = WR.numel() // ncomp_trans
+        self.plan._bind(WR, d_cor)
+        _check(lib().sht_sum_coriolis_rlm(
+            self._h, ncomp_trans, i_velo, i_vort, float(coef_cor), _ptr(WR), nnod_recv,
+            irev.data_ptr() if irev is not None else None, _ptr(d_cor)), "sht_sum_coriolis_rlm")
+
+    def copy_coriolis_terms_rlm(self, ncomp_trans: int, i_rot_coriolis: int, d_cor, WS,
+                                item=None):
+        """Both parts of d_cor to components i_rot_coriolis and +2 of WS."""
+        nnod_send = WS.numel() // ncomp_trans
+        self.plan._bind(d_cor, WS)
+        _check(lib().sht_copy_coriolis_terms_rlm(
+            self._h, ncomp_trans, i_rot_coriolis, _ptr(d_cor),
+            item.data_ptr() if item is not None else None, nnod_send, _ptr(WS)),
+            "sht_copy_coriolis_terms_rlm")
 
 
 class ShellBatch:

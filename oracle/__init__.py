@@ -66,6 +66,9 @@ def lib() -> ctypes.CDLL:
                                      _ip, ctypes.c_longlong, ctypes.c_int, _dp]
         L.orc_rj_from_recv.argtypes = [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_longlong, _dp,
                                        _ip, ctypes.c_int, ctypes.c_longlong, _dp]
+        L.orc_coriolis_tables.argtypes = [ctypes.c_void_p, _ip, _ip, _dp, _dp, _dp, _dp]
+        L.orc_sum_coriolis_rlm.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_double, _dp, _ip, ctypes.c_int, _dp]
         L.orc_num_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -294,6 +297,33 @@ class OraclePlan:
         ca = np.array([coef.get(k, 0.0) for k in self.NL_OUT], dtype=np.float64)
         lib().orc_nonlinear_rtp(self._h, ia, _ptr(ca), ncomp_fld, _ptr(fld), ncomp_frc, _ptr(out))
         return out
+
+    def coriolis_tables(self):
+        """Gaunt / Elsasser integrals and the Coriolis coefficients in the
+        reference's shapes: jgi (2, jmax) 1-based, jei (jmax), gi (2, jmax),
+        ei (jmax), sw (jmax, 3, 2), tw (jmax, 4, 2) (C order of sw(2,3,j),
+        tw(2,4,j))."""
+        jm = self.jmax
+        jgi = np.zeros(2 * jm, dtype=np.int32)
+        jei = np.zeros(jm, dtype=np.int32)
+        gi, ei = np.zeros(2 * jm), np.zeros(jm)
+        sw, tw = np.zeros(6 * jm), np.zeros(8 * jm)
+        ip = ctypes.POINTER(ctypes.c_int)
+        lib().orc_coriolis_tables(self._h, jgi.ctypes.data_as(ip), jei.ctypes.data_as(ip),
+                                  _ptr(gi), _ptr(ei), _ptr(sw), _ptr(tw))
+        return dict(jgi=jgi.reshape(2, jm), jei=jei, gi=gi.reshape(2, jm), ei=ei,
+                    sw=sw.reshape(jm, 3, 2), tw=tw.reshape(jm, 4, 2))
+
+    def sum_coriolis_rlm(self, ncomp, i_velo, i_vort, coef_cor, WR, irev=None, kr_icb=0):
+        """Curl of the Coriolis term in rlm space: (2, jmax*nr) array of the
+        poloidal and toroidal parts (i_velo, i_vort 0-based components)."""
+        WR = np.ascontiguousarray(WR, dtype=np.float64).ravel()
+        assert WR.size >= self.n_rj(ncomp)
+        ta, tp = _table(irev)
+        out = np.zeros(2 * self.jmax * self.nr)
+        lib().orc_sum_coriolis_rlm(self._h, ncomp, i_velo, i_vort, coef_cor, _ptr(WR), tp,
+                                   kr_icb, _ptr(out))
+        return out.reshape(2, -1)
 
     def forward(self, nv, ns, v_rtp, variant="sym"):
         ncomp = 3 * nv + ns

@@ -45,6 +45,8 @@ def lib():
         _ip = ctypes.POINTER(ctypes.c_int)
         L.ref_to_send_buf.argtypes = [ctypes.c_int] * 4 + [_ip] + [ctypes.c_int] * 3 + [_dp, _dp]
         L.ref_from_recv_rev.argtypes = [ctypes.c_int] * 4 + [_ip] + [ctypes.c_int] * 3 + [_dp, _dp]
+        L.ref_coriolis_tables.argtypes = [_ip, _ip] + [_dp] * 8
+        L.ref_sum_coriolis.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double, ctypes.c_int, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -157,6 +159,29 @@ class RefPlan:
         WS = np.zeros(ncomp * self.jmax * self.nr)
         lib().ref_leg_fwd(nv, ns, _p(WR), _p(WS))
         return WS
+
+    def coriolis_tables(self):
+        """cal_gaunt_coriolis_rlm + interact_rot_coriolis_rlm on this plan's
+        rlm order, in the layout of oracle.OraclePlan.coriolis_tables."""
+        jm = self.jmax
+        ip = ctypes.POINTER(ctypes.c_int)
+        jgi = np.zeros(2 * jm, dtype=np.int32)
+        jei = np.zeros(jm, dtype=np.int32)
+        gi, ei = np.zeros(2 * jm), np.zeros(jm)
+        sw, tw, sd = np.zeros(6 * jm), np.zeros(8 * jm), np.zeros(4 * jm)
+        td, sr, tr = np.zeros(2 * jm), np.zeros(2 * jm), np.zeros(2 * jm)
+        lib().ref_coriolis_tables(jgi.ctypes.data_as(ip), jei.ctypes.data_as(ip), _p(gi), _p(ei),
+                                  _p(sw), _p(tw), _p(sd), _p(td), _p(sr), _p(tr))
+        return dict(jgi=jgi.reshape(2, jm), jei=jei, gi=gi.reshape(2, jm), ei=ei,
+                    sw=sw.reshape(jm, 3, 2), tw=tw.reshape(jm, 4, 2))
+
+    def sum_coriolis(self, ncomp, i_velo, i_vort, coef_cor, sp, kr_icb=0):
+        """sum_rot_coriolis_rlm_10 (+ inner_core_rot_z_coriolis_rlm when
+        kr_icb > 0, 1-based); i_velo / i_vort 0-based.  (2, jmax*nr)."""
+        WR = np.ascontiguousarray(sp, dtype=np.float64).ravel().copy()
+        out = np.zeros(2 * self.jmax * self.nr)
+        lib().ref_sum_coriolis(ncomp, i_velo + 1, i_vort + 1, coef_cor, kr_icb, _p(WR), _p(out))
+        return out.reshape(2, -1)
 
     def center(self, nv, ns, sp, v_pole):
         """The reference's centre row (SS/schmidt_b_trans_at_center.f90)

@@ -28,6 +28,10 @@
       use cal_sp_rlm_by_matmul
       use schmidt_b_trans_at_center
       use field_to_send_buffer
+      use t_phys_address
+      use t_gaunt_coriolis_rlm
+      use interact_coriolis_rlm
+      use sum_coriolis_terms_rlm
 !
       implicit none
 !
@@ -297,6 +301,97 @@
       end select
       deallocate(irev)
       end subroutine ref_from_recv_rev
+!
+!> Coriolis tables of init_sum_coriolis_rlm
+!! (MS/t_coriolis_terms_rlm.f90:133-183) on the tables of ref_setup:
+!! cal_gaunt_coriolis_rlm and interact_rot_coriolis_rlm
+!! (MS/interact_coriolis_rlm.f90), MS/ = MHD/Fortran_src/MHD_src/sph_MHD/.
+!! Output arrays in the reference's shapes (first index fastest).
+      subroutine ref_coriolis_tables(jgi, jei, gi, ei, sw, tw, sd, td,  &
+     &          sr, tr) bind(C, name='ref_coriolis_tables')
+      integer(c_int), intent(inout) :: jgi(jmax_h,2), jei(jmax_h,1)
+      real(c_double), intent(inout) :: gi(jmax_h,2), ei(jmax_h,1)
+      real(c_double), intent(inout) :: sw(2,3,jmax_h), tw(2,4,jmax_h)
+      real(c_double), intent(inout) :: sd(2,2,jmax_h), td(2,jmax_h)
+      real(c_double), intent(inout) :: sr(2,jmax_h), tr(2,jmax_h)
+      integer(kind = kint), allocatable :: jg(:,:), je(:,:)
+!
+      allocate(jg(jmax_h,2), je(jmax_h,1))
+      jg = 0
+      je = 0
+      gi = 0.0d0
+      ei = 0.0d0
+      sw = 0.0d0
+      tw = 0.0d0
+      sd = 0.0d0
+      td = 0.0d0
+      sr = 0.0d0
+      tr = 0.0d0
+      call cal_gaunt_coriolis_rlm(ltr_h, jmax_h, idx_j_h, jg, je,       &
+     &    gi, ei)
+      call interact_rot_coriolis_rlm(jmax_h, g_h, gi, ei,               &
+     &    sw, tw, sd, td, sr, tr)
+      jgi(1:jmax_h,1:2) = jg(1:jmax_h,1:2)
+      jei(1:jmax_h,1) = je(1:jmax_h,1)
+      deallocate(jg, je)
+      end subroutine ref_coriolis_tables
+!
+!> sum_rot_coriolis_rlm_10 and, for kr_icb > 0, inner_core_rot_z_coriolis_rlm
+!! (MS/sum_coriolis_terms_rlm.f90) on the tables of ref_setup, as
+!! sum_coriolis_rlm (MS/t_coriolis_terms_rlm.f90:187-258) calls them:
+!! i_velo / i_vort 1-based components of WR (NB = ncomp), irev_sr_rlm the
+!! identity, omega_rlm as set_zdir_rot_earth_4_sph with rotate = 1
+!! (MS/t_poloidal_rotation.f90:155-250: r^2/2, r, 1).  d_cor(nnod_rlm,2).
+      subroutine ref_sum_coriolis(ncomp, i_velo, i_vort, coef_cor,      &
+     &          kr_icb, WR, d_cor) bind(C, name='ref_sum_coriolis')
+      integer(c_int), value :: ncomp, i_velo, i_vort, kr_icb
+      real(c_double), value :: coef_cor
+      real(c_double), intent(in) :: WR(*)
+      real(c_double), intent(inout) :: d_cor(jmax_h*nr_h,2)
+      type(phys_address) :: b_trns
+      integer(kind = kint) :: nnod, n_WR, k, j, nbk, m
+      integer(kind = kint) :: jgi(jmax_h,2), jei(jmax_h,1)
+      integer(kind = kint) :: idx1(-1:1)
+      integer(kind = kint), allocatable :: irev(:)
+      real(kind = kreal) :: gi(jmax_h,2), ei(jmax_h,1)
+      real(kind = kreal), allocatable :: sw(:,:,:), tw(:,:,:)
+      real(kind = kreal), allocatable :: sd(:,:,:), td(:,:)
+      real(kind = kreal), allocatable :: sr(:,:), tr(:,:)
+      real(kind = kreal), allocatable :: omega(:,:), rj(:)
+!
+      nnod = jmax_h * nr_h
+      nbk = ncomp
+      n_WR = nbk * nnod
+      allocate(irev(nnod), omega(nr_h,0:2), rj(max(jmax_h, nr_h)))
+      allocate(sw(2,3,jmax_h), tw(2,4,jmax_h), sd(2,2,jmax_h))
+      allocate(td(2,jmax_h), sr(2,jmax_h), tr(2,jmax_h))
+      do j = 1, nnod
+        irev(j) = j
+      end do
+      do k = 1, nr_h
+        omega(k,0) = half * one * radius_h(k)**2
+        omega(k,1) =        one * radius_h(k)
+        omega(k,2) =        one
+      end do
+      call ref_coriolis_tables(jgi, jei, gi, ei, sw, tw, sd, td, sr, tr)
+      b_trns%i_velo = i_velo
+      b_trns%i_vort = i_vort
+      d_cor = 0.0d0
+      call sum_rot_coriolis_rlm_10(b_trns, nnod, nidx_rlm_h, a_r_h,     &
+     &    g_h, omega, coef_cor, jgi, jei, sw, tw, nbk, n_WR, irev, WR,  &
+     &    d_cor(1,1), d_cor(1,2))
+      if(kr_icb .gt. 0) then
+        do m = -1, 1
+          idx1(m) = find_local_sph_rlm_address(jmax_h, idx_j_h, ione, m)
+        end do
+        rj = 0.0d0
+        rj(1:nr_h) = radius_h(1:nr_h)
+        call inner_core_rot_z_coriolis_rlm(b_trns, nnod, nidx_rlm_h,    &
+     &      rj, omega, coef_cor, nbk, n_WR, irev, WR,                   &
+     &      int(kr_icb, kind = kint), idx1, d_cor(1,1))
+      end if
+      deallocate(irev, omega, rj, sw, tw, sd, td, sr, tr)
+      end subroutine ref_sum_coriolis
 !
       subroutine ref_release() bind(C, name='ref_release')
       if(allocated(radius_h)) deallocate(radius_h, a_r_h)

@@ -1317,3 +1317,174 @@ void orc_rj_from_recv(int nfield, const int *fields, int nb, long long nnod_recv
     }
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* Coriolis term in rlm space (MS/ = MHD/Fortran_src/MHD_src/sph_MHD/).
+ *
+ * Tables of init_sum_coriolis_rlm (MS/t_coriolis_terms_rlm.f90:133-183):
+ * cal_gaunt_coriolis_rlm (MS/interact_coriolis_rlm.f90:42-118) takes, for
+ * each rlm j = (l, m), the Gaunt integrals
+ *   gi(j,1) = K(1,0; l-1,m; l,m)  (0 unless 1 <= l-1 and |m| <= l-1),
+ *   gi(j,2) = K(1,0; l+1,m; l,m)  (0 unless l+1 <= L),
+ * and the Elsasser integral ei(j) = L(1,0; l,-m; l,m), with the rlm
+ * address of the partner mode (the mode itself where the integral is 0;
+ * all partners (0,0) for j = 0).  The reference evaluates them from prime
+ * factorisations (SH/normgi_by_primes.f90, SH/normei.f90,
+ * SH/cal_gaunt_itgs.f90:84-164); for these index sets they reduce, with the
+ * Schmidt recurrence x P_l^m = (sqrt((l+1)^2-m^2) P_(l+1)^m +
+ * sqrt(l^2-m^2) P_(l-1)^m)/(2l+1), int (P_l^m)^2 dx = 2(2-d_m0)/(2l+1) and
+ * normki's factor 2 for m = 0, to
+ *   gi(j,1) = 4 sqrt(l^2 - m^2) / ((2l-1)(2l+1)),
+ *   gi(j,2) = 4 sqrt((l+1)^2 - m^2) / ((2l+1)(2l+3)),
+ *   ei(j)   = -4 m / (2l+1)  (l >= 1; m = 0 gives 0),
+ * the sign being that of setei's combination of Gaunt integrals
+ * (SH/normei.f90:76-150) with normli's for m2 = -m3 (tests pin all three
+ * against the routines compiled from source: addresses equal, values within
+ * a few ulp, since the reference forms them from products of square roots).
+ * interact_rot_coriolis_rlm (:122-205) then forms sw(2,3,j) and tw(2,4,j)
+ * from these and g3, g4, g5, g17 (g3(0,0) = 1/2), in the reference's
+ * operation order.  jgi[j + jmax*c], jei[j] are 1-based as in the
+ * reference; sw[a + 2*(b + 3*j)], tw[a + 2*(b + 4*j)]. */
+void orc_coriolis_tables(const orc_plan *p, int *jgi, int *jei, double *gi, double *ei,
+                         double *sw, double *tw) {
+  const int jmax = p->jmax, L = p->L;
+  const double two = 2.0, half = 0.5, zero = 0.0;
+  int j;
+  /* local rlm address of (l, m) in the single-rank order: slot s = m + L
+   * holds l = |m|..L from lstack[s] */
+#define ORC_JADDR(l, m) (p->lstack[(m) + L] + ((l) - abs(m)) + 1)
+  for (j = 0; j < jmax; j++) {
+    const int l = p->idx_j[j + jmax], m = p->idx_j[j + 2 * jmax];
+    const int j3 = p->idx_j[j];
+    double g[17], g3, g4, g5, g17, gi1 = zero, gi2 = zero, e1 = zero;
+    int a1 = ORC_JADDR(l, m), a2 = ORC_JADDR(l, m), ae = ORC_JADDR(l, m);
+    if (l - 1 >= 1 && abs(m) <= l - 1) {
+      gi1 = 4.0 * sqrt((double)(l * l - m * m)) / ((double)(2 * l - 1) * (double)(2 * l + 1));
+      a1 = ORC_JADDR(l - 1, m);
+    }
+    if (l + 1 <= L) {
+      gi2 = 4.0 * sqrt((double)((l + 1) * (l + 1) - m * m)) /
+            ((double)(2 * l + 1) * (double)(2 * l + 3));
+      a2 = ORC_JADDR(l + 1, m);
+    }
+    if (l >= 1) {
+      if (m != 0) e1 = -4.0 * (double)m / (double)(2 * l + 1);
+      ae = ORC_JADDR(l, -m);
+    }
+    if (j3 == 0) {
+      a1 = a2 = ae = ORC_JADDR(0, 0);
+      gi2 = zero;
+      e1 = zero;
+    }
+    jgi[j] = a1;
+    jgi[j + jmax] = a2;
+    jei[j] = ae;
+    gi[j] = gi1;
+    gi[j + jmax] = gi2;
+    ei[j] = e1;
+    orc_sph_normalizations(l, m, g);
+    g3 = (l == 0 && m == 0) ? half : g[2];
+    g4 = g[3];
+    g5 = g[4];
+    g17 = g[16];
+    {
+      double *s = sw + 6 * (long long)j, *t = tw + 8 * (long long)j;
+      s[0] = (two - g4 - g3) * gi1 * g17;
+      s[1] = (two - g5 - g3) * gi2 * g17;
+      s[2] = g4 * (two - g4 + g3) * gi1 * g17 * half;
+      s[3] = g5 * (two - g5 + g3) * gi2 * g17 * half;
+      s[4] = two * e1 * g17;
+      s[5] = zero;
+      t[4] = -(g3 * (two + g4 - g3) + two * (-two + g4 + g3)) * gi1 * g17 * half;
+      t[5] = -(g3 * (two + g5 - g3) + two * (-two + g5 + g3)) * gi2 * g17 * half;
+      t[6] = -(-two + g4 + g3) * gi1 * g17;
+      t[7] = -(-two + g5 + g3) * gi2 * g17;
+      t[0] = -two * e1 * g17;
+      t[1] = zero;
+      t[2] = -g3 * e1 * g17;
+      t[3] = zero;
+    }
+  }
+#undef ORC_JADDR
+}
+
+/* sum_rot_coriolis_rlm_10 (MS/sum_coriolis_terms_rlm.f90:96-186): curl of
+ * the Coriolis force of a rotation along z, poloidal (d_cor[i]) and toroidal
+ * (d_cor[i + nnod_rlm]) parts per rlm node i = j + jmax*k, from velocity and
+ * vorticity spectra (pol, dpol/dr, tor) at components i_velo, i_vort
+ * (0-based) of the comp-interleaved receive buffer WR (record irev[i]-1;
+ * irev NULL = identity; the dummy record irev = nnod_rlm + 1 reads zero).  The rotation spectrum is that of
+ * set_zdir_rot_earth_4_sph with rotate = 1 (MS/t_poloidal_rotation.f90:
+ * 159-250): omega0 = r^2/2, omega1 = r, omega2 = 1.  kr_icb > 0 (1-based
+ * radius) adds inner_core_rot_z_coriolis_rlm (:378-418) for a rotatable
+ * inner core.  Operation order as in the reference. */
+void orc_sum_coriolis_rlm(const orc_plan *p, int ncomp, int i_velo, int i_vort, double coef_cor,
+                          const double *WR, const int *irev, int kr_icb, double *d_cor) {
+  const int jmax = p->jmax, nr = p->nr;
+  const long long nnod = (long long)jmax * nr;
+  int *jgi = (int *)malloc(sizeof(int) * 3 * jmax);
+  int *jei = jgi + 2 * jmax;
+  double *gi = (double *)malloc(sizeof(double) * 17 * jmax);
+  double *ei = gi + 2 * jmax, *sw = ei + jmax, *tw = sw + 6 * jmax;
+  const double two = 2.0, half = 0.5;
+  int k;
+  /* record r of the receive buffer; the dummy record r = nnod (irev =
+   * nnod_rlm + 1) reads zero, as the reference zeroes it
+   * (SR/set_from_recv_buf_rev.f90:77-87) */
+#define ORC_WR(c, r) ((r) < 0 || (r) >= nnod ? 0.0 : WR[(c) + (long long)ncomp * (r)])
+  orc_coriolis_tables(p, jgi, jei, gi, ei, sw, tw);
+#pragma omp parallel for schedule(static)
+  for (k = 0; k < nr; k++) {
+    const double ar = p->a_r[k], r = p->radius[k];
+    const double om0 = half * 1.0 * (r * r), om1 = 1.0 * r, om2 = 1.0;
+    int j;
+    for (j = 0; j < jmax; j++) {
+      const long long i = j + (long long)jmax * k;
+      const long long i11 = jgi[j] - 1 + (long long)jmax * k;
+      const long long i21 = jgi[j + jmax] - 1 + (long long)jmax * k;
+      const long long i12 = jei[j] - 1 + (long long)jmax * k;
+      const long long r11 = irev ? irev[i11] - 1 : i11;
+      const long long r21 = irev ? irev[i21] - 1 : i21;
+      const long long r12 = irev ? irev[i12] - 1 : i12;
+      const double *s = sw + 6 * (long long)j, *t = tw + 8 * (long long)j;
+      const double dvp_k1 = ORC_WR(i_velo + 1, r11);
+      const double dvp_k2 = ORC_WR(i_velo + 1, r21);
+      const double wp_l1 = ORC_WR(i_vort, r12);
+      const double vp_k1 = ORC_WR(i_velo, r11);
+      const double vp_k2 = ORC_WR(i_velo, r21);
+      const double d2vp_l1 = ar * ar * p->g3[j] * ORC_WR(i_velo, r12) -
+                             ORC_WR(i_vort + 2, r12);
+      const double vp_l1 = ORC_WR(i_velo, r12);
+      const double wp_k1 = ORC_WR(i_vort, r11);
+      const double wp_k2 = ORC_WR(i_vort, r21);
+      const double dwp_k1 = ORC_WR(i_vort + 1, r11) - two * ar * ORC_WR(i_vort, r11);
+      const double dwp_k2 = ORC_WR(i_vort + 1, r21) - two * ar * ORC_WR(i_vort, r21);
+      double rp = s[0] * om0 * dvp_k1 + s[1] * om0 * dvp_k2 + s[4] * om0 * wp_l1 +
+                  s[2] * om1 * vp_k1 + s[3] * om1 * vp_k2;
+      double rt = t[0] * om0 * d2vp_l1 - t[2] * om2 * vp_l1 + t[4] * om1 * wp_k1 +
+                  t[5] * om1 * wp_k2 + t[6] * om0 * dwp_k1 + t[7] * om0 * dwp_k2;
+      d_cor[i] = -coef_cor * rp * ar * ar;
+      d_cor[i + nnod] = -coef_cor * rt * ar * ar;
+    }
+  }
+  if (kr_icb > 0 && kr_icb <= nr && p->L >= 1) {
+    /* inner_core_rot_z_coriolis_rlm: degree-one poloidal terms at the ICB */
+    const int L = p->L;
+    const long long ko = (long long)jmax * (kr_icb - 1);
+    const long long i11s = p->lstack[-1 + L] + (1 - 1) + ko; /* (1, -1) */
+    const long long i10c = p->lstack[0 + L] + (1 - 0) + ko;  /* (1, 0) */
+    const long long i11c = p->lstack[1 + L] + (1 - 1) + ko;  /* (1, 1) */
+    const long long r11s = irev ? irev[i11s] - 1 : i11s;
+    const long long r11c = irev ? irev[i11c] - 1 : i11c;
+    const double rc = p->radius[kr_icb - 1];
+    const double om0 = half * 1.0 * (rc * rc);
+    const double wp11s = ORC_WR(i_vort, r11s);
+    const double wp11c = ORC_WR(i_vort, r11c);
+    d_cor[i10c] = 0.0;
+    d_cor[i11s] = -two * coef_cor * rc * om0 * wp11c;
+    d_cor[i11c] = two * coef_cor * rc * om0 * wp11s;
+  }
+#undef ORC_WR
+  free(jgi);
+  free(gi);
+}
