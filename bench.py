@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--shells", type=int, default=0,
                     help="only run the batched-shell workload: this many seeded uneven shells, "
                          "one HIP graph launch against a loop of per-shell calls")
+    ap.add_argument("--variant", action="append", default=[], metavar="NAME=VALUE",
+                    help="kernel variant for every plan (sht_plan_set_variant), repeatable")
     ap.add_argument("--nonlinear", action="store_true",
                     help="only time the grid-space products of one MHD step (all six terms) "
                          "on the configuration's grid")
@@ -145,6 +147,7 @@ def run_shells(args, torch):
                 p = SphTransPlan(c, max_ncomp=4, precision=SHT_PREC_SPLIT_INT8 if prec == "split"
                                  else SHT_PREC_FP64, n_slices=args.slices)
                 p.set_stream(stream.cuda_stream)
+                apply_variants(p, args)
                 ws = torch.randn(c.n_rj(4), generator=gen, dtype=torch.float64, device="cuda")
                 vin = torch.randn(c.n_rtp(4), generator=gen, dtype=torch.float64, device="cuda")
                 vout = torch.empty(c.n_rtp(4), dtype=torch.float64, device="cuda")
@@ -192,6 +195,12 @@ def flops_per_contraction(c):
     return 2.0 * ((c["nth"] + 1) // 2) * (c["L"] + 1) ** 2 * c["nr"]
 
 
+def apply_variants(plan, args):
+    for kv in args.variant:
+        name, value = kv.split("=")
+        plan.set_variant(name, int(value))
+
+
 def make_config(cfg_d):
     from kemorin_mhd_amd import ShellConfig
     return ShellConfig(cfg_d["L"], cfg_d["nr"], cfg_d["nth"], cfg_d["nphi"])
@@ -234,6 +243,7 @@ def run_engine(cfg, precision, args, torch, sp_b, grid_f, keep=False):
     torch.cuda.synchronize()
     plan = SphTransPlan(cfg, max_ncomp=9, precision=prec, n_slices=args.slices)
     plan.set_stream(stream.cuda_stream)
+    apply_variants(plan, args)
     dev = "cuda"
     with torch.cuda.stream(stream):
         grid_b = torch.empty(cfg.n_rtp(8), dtype=torch.float64, device=dev)
@@ -492,7 +502,7 @@ def main():
                                 f"Ntheta={cfg_d['nth']} Nphi={cfg_d['nphi']}: backward 2 vector + "
                                 f"2 scalar (8 comps), forward 3 vector (9 comps)"),
                    "engine": head, "L": cfg_d["L"], "nr": cfg_d["nr"], "ntheta": cfg_d["nth"],
-                   "nphi": cfg_d["nphi"]},
+                   "nphi": cfg_d["nphi"], "variants": args.variant},
         "roofline": roof,
         "gflops_fp64_equivalent": F_rt / (res["ms_per_step"] / 1e3) / 1e9,
         "stages": per_stage,

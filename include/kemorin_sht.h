@@ -23,7 +23,9 @@
  * (hipMalloc'd or torch CUDA tensors) unless the name ends in _host.
  * Calls on one plan are stream-ordered on the plan's stream and not
  * re-entrant; different plans may run concurrently.  A plan's own stream is
- * a blocking stream, ordered with the legacy default stream.
+ * a blocking stream; a caller whose inputs are written by work queued on the
+ * legacy default stream (asynchronous copies included) binds the plan to
+ * hipStreamLegacy with sht_plan_set_stream, so that both share one queue.
  */
 #ifndef KEMORIN_SHT_H
 #define KEMORIN_SHT_H
@@ -64,13 +66,26 @@ typedef struct {
  * index stacks and FFT twiddles.  The plan owns all device memory. */
 int sht_plan_create(const sht_desc *desc, sht_plan **plan);
 int sht_plan_destroy(sht_plan *plan);
-/* Run subsequent work on this hipStream_t (NULL = plan's own stream). */
+/* Run subsequent work on this hipStream_t (NULL = plan's own stream;
+ * hipStreamLegacy = the legacy default stream, for callers whose other work
+ * is queued there). */
 int sht_plan_set_stream(sht_plan *plan, void *hip_stream);
 /* Copy a plan table to host: "gpoint","weight","colat","asin_t","radius"
  * (double) or "lstack" (int, 2L+2 entries).  n = capacity in elements. */
 int sht_plan_get_table(const sht_plan *plan, const char *name, void *out, long long n);
 /* Device bytes held by the plan (tables + workspace). */
 long long sht_plan_device_bytes(const sht_plan *plan);
+/* Kernel variants (tile shapes) of a plan, for comparisons and tuning.  The
+ * reference's closest counterpart is its choice of Legendre loop by timing
+ * at start-up (MS/init_sphrical_transform_MHD.f90:201-345,
+ * MS/ = MHD/Fortran_src/MHD_src/sph_MHD/).  Names and values:
+ *   "split_fwd_mfma": 16 or 32, the INT8 MFMA form of the forward split
+ *                     Legendre products (v_mfma_i32_16x16x64_i8 or
+ *                     v_mfma_i32_32x32x32_i8).
+ * Results do not depend on the variant.  SHT_ERR_ARG for an unknown name or
+ * value. */
+int sht_plan_set_variant(sht_plan *plan, const char *name, int value);
+int sht_plan_get_variant(const sht_plan *plan, const char *name, int *value);
 
 /* Stage timers, the counterpart of the reference's elapsed-time ids for the
  * transform (MHD/Fortran_src/MHD_src/IO/init_sph_MHD_elapsed_label.f90:59-65:

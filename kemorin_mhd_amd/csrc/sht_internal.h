@@ -134,6 +134,9 @@ struct sht_plan {
   long long device_bytes = 0;
   // split-precision state (allocated only in SHT_PREC_SPLIT_INT8 mode)
   void *split = nullptr;
+  // kernel variants (sht_plan_set_variant): INT8 MFMA form of the forward
+  // split GEMM, 16 (16x16x64) or 32 (32x32x32)
+  int var_fwd_mfma = 16;
   // stage timers (sht_plan_timer_*): event pairs recorded per launch
   int timing = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> tev[ksht::kNumStages];

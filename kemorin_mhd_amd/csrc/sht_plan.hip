@@ -384,6 +384,54 @@ extern "C" int sht_plan_set_stream(sht_plan *p, void *s) {
 
 extern "C" long long sht_plan_device_bytes(const sht_plan *p) { return p ? p->device_bytes : 0; }
 
+// Kernel variants: name -> the plan field and its allowed values.
+static int *variant_slot(sht_plan *p, const char *name, const int **allowed, int *nallowed) {
+  static const int mfma[] = {16, 32};
+  if (!strcmp(name, "split_fwd_mfma")) {
+    *allowed = mfma;
+    *nallowed = 2;
+    return &p->var_fwd_mfma;
+  }
+  return nullptr;
+}
+
+extern "C" int sht_plan_set_variant(sht_plan *p, const char *name, int value) {
+  if (!p || !name) {
+    set_error("sht_plan_set_variant: null argument");
+    return SHT_ERR_ARG;
+  }
+  const int *allowed = nullptr;
+  int n = 0;
+  int *slot = variant_slot(p, name, &allowed, &n);
+  if (!slot) {
+    set_error("sht_plan_set_variant: unknown variant '%s'", name);
+    return SHT_ERR_ARG;
+  }
+  for (int i = 0; i < n; i++)
+    if (allowed[i] == value) {
+      *slot = value;
+      return SHT_OK;
+    }
+  set_error("sht_plan_set_variant: %s = %d is not an allowed value", name, value);
+  return SHT_ERR_ARG;
+}
+
+extern "C" int sht_plan_get_variant(const sht_plan *p, const char *name, int *value) {
+  if (!p || !name || !value) {
+    set_error("sht_plan_get_variant: null argument");
+    return SHT_ERR_ARG;
+  }
+  const int *allowed = nullptr;
+  int n = 0;
+  int *slot = variant_slot(const_cast<sht_plan *>(p), name, &allowed, &n);
+  if (!slot) {
+    set_error("sht_plan_get_variant: unknown variant '%s'", name);
+    return SHT_ERR_ARG;
+  }
+  *value = *slot;
+  return SHT_OK;
+}
+
 extern "C" int sht_plan_get_table(const sht_plan *p, const char *name, void *out, long long n) {
   if (!p || !name || !out) {
     set_error("sht_plan_get_table: null argument");

@@ -531,6 +531,17 @@ __device__ __forceinline__ i4 load_slice(const void *base, unsigned voff) {
   asm volatile("global_load_dwordx4 %0, %1, %2" : "=v"(v) : "v"(voff), "s"(uniform_ptr(base)) : "memory");
   return v;
 }
+// load_slice with the five wait states of a base SGPR just written by a VALU
+// instruction (the 32x32 kernel's bases can come back from SGPR spill lanes
+// through v_readlane right before the load)
+__device__ __forceinline__ i4 load_slice_ws(const void *base, unsigned voff) {
+  i4 v;
+  asm volatile("s_nop 4\n\tglobal_load_dwordx4 %0, %1, %2"
+               : "=v"(v)
+               : "v"(voff), "s"(uniform_ptr(base))
+               : "memory");
+  return v;
+}
 template <int N>
 __device__ __forceinline__ void wait_vm() {
   static_assert(N >= 0 && N < 64, "vmcnt range");
@@ -1128,6 +1139,288 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
 #undef KSHT_SLICED_PRODUCTS
 
 // ---------------------------------------------------------------------------
+// Forward on v_mfma_i32_32x32x32_i8.  On MI355X this form sustains about 1.7x
+// the INT8 rate of the 16x16x64 form (measured with independent accumulator
+// chains: 4.0 against 2.3 POP/s at the occupancies these kernels run at), so
+// the same products take a 32 x 32 output tile per instruction: 32 degree
+// rows of one parity by the 16 radii of BOTH slots +-M (the two slots share
+// the P slices, so they sit side by side in N).  A work-group of 4 waves
+// (one per SIMD; the tile's accumulators need the whole register file)
+// covers the same 64-row block of both parities as the 16x16 kernel and
+// keeps its LDS layout, copies and exponents: only the lanes' fragment reads
+// change.
+//   A (32 rows x 32 K): lane l holds row l%32, K 16(l/32) .. +15 of a K half;
+//   B (32 K x 32 cols): lane l holds column l%32 (slot (l/16)%2, radius l%16),
+//     the same K bytes; read from the slot's 1 KiB LDS block, whose 16 bytes
+//     of (radius r, K 16 g .. +15) sit at 16 (r + 16 g);
+//   D: lane l, element e: column l%32, row 8 (e/4) + 4 (l/32) + e%4.
+// A 64-wide exponent chunk is two K halves into the same INT32 accumulators
+// (the bounds of the 16x16 form hold: the sum still runs over 64 products).
+template <int S, int NB>
+struct FwdLds32 {
+  static constexpr int W = 4;             // waves
+  static constexpr int NBB = 4 * NB * S;  // [slot][sym/asym][component][digit]
+  static constexpr int NET = NB + 6;
+  static constexpr int KB = (NBB + W - 1) / W, KE = (NET + W - 1) / W;
+  static constexpr int EX = W * KB * 64;       // i4 offset of the exponents
+  static constexpr int I4 = EX + W * KE * 16;  // 64 ints per exponent task
+  static constexpr int NBUF = (2 * (I4 * 16) <= 160 * 1024) ? 2 : 1;
+  static constexpr int bytes() { return NBUF * I4 * 16; }
+};
+constexpr int kGemm32Threads = 256;
+
+typedef int i16 __attribute__((ext_vector_type(16)));
+__device__ __forceinline__ i16 mfma32(i4 a, i4 b, i16 c) {
+  return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0);
+}
+
+// Triangular sliced products of quantity Q: A slices AOF(s, kh), data slices
+// BOF(t, kh).  Every digit group g = s + t < S has its own INT32 accumulator,
+// so each data slice feeds up to S independent MFMA chains back to back; the
+// groups are then folded pairwise into FP64, T[e] = sum of pair value times
+// pair_scale, from the smallest scale up (as in the 16x16 form, so the sums
+// round identically).
+#define KSHT_SLICED32(Q, T)                                                              \
+  {                                                                                      \
+    i16 acc_[S];                                                                         \
+    _Pragma("unroll") for (int g_ = 0; g_ < S; g_++)                                     \
+      _Pragma("unroll") for (int e_ = 0; e_ < 16; e_++) acc_[g_][e_] = 0;                \
+    _Pragma("unroll") for (int t = 0; t < S; t++) {                                      \
+      _Pragma("unroll") for (int kh = 0; kh < 2; kh++) {                                 \
+        const i4 b = BOF(Q, t, kh);                                                      \
+        _Pragma("unroll") for (int s_ = 0; s_ + t < S; s_++)                             \
+          acc_[s_ + t] = mfma32(AOF(s_, kh), b, acc_[s_ + t]);                           \
+      }                                                                                  \
+    }                                                                                    \
+    _Pragma("unroll") for (int e_ = 0; e_ < 16; e_++) T[e_] = 0.0;                       \
+    _Pragma("unroll") for (int gg = 0; gg < (S + 1) / 2; gg++) {                         \
+      const int g0 = 2 * ((S + 1) / 2 - 1 - gg);                                         \
+      const bool two = g0 + 1 < S;                                                       \
+      const double sc = pair_scale(g0, two);                                             \
+      _Pragma("unroll") for (int e_ = 0; e_ < 16; e_++) {                                \
+        const int v = two ? acc_[g0][e_] * 256 + acc_[two ? g0 + 1 : g0][e_] : acc_[g0][e_]; \
+        T[e_] = fma((double)v, sc, T[e_]);                                               \
+      }                                                                                  \
+    }                                                                                    \
+  }
+
+template <int S, int NB, bool VEC>
+__global__ void __launch_bounds__(kGemm32Threads) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_leg_fwd_split32(
+    SplitDims d, int nd_first, SplitTabs tb, const signed char *__restrict__ dig,
+    const int *__restrict__ dexp, double *__restrict__ sp, SpecAddr sa,
+    const int *__restrict__ jst, const int *__restrict__ ipad, const double *__restrict__ radius) {
+  extern __shared__ i4 smem[];
+  using LY = FwdLds32<S, NB>;
+  constexpr int NBUF = LY::NBUF;
+  constexpr int W = LY::W;
+  constexpr int NO = VEC ? 3 : NB;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform per wave
+  // same logical order as the 16x16 kernel: runs of (M, row block)
+  long long bidx = xcd_group_order(blockIdx.x, gridDim.x, (long long)d.n_rt * d.ncg);
+  const int cgl = (int)(bidx % d.ncg);
+  bidx /= d.ncg;
+  const int rt = (int)(bidx % d.n_rt);
+  bidx /= d.n_rt;
+  const int M = (int)(bidx % (d.L + 1));
+  const int ib = (int)(bidx / (d.L + 1));
+  const int L = d.L;
+  const int ip0 = ipad[2 * M], ip1 = ipad[2 * M + 1];
+  if (ib * 64 >= ip0) return;  // whole work-group beyond the degrees of M
+  const int nsig = M == 0 ? 1 : 2;
+  // wave w: parity w & 1, 32-row tile w >> 1
+  const int par = wave & 1, tile = wave >> 1;
+  const int i0 = ib * 64 + tile * 32;
+  const int ip = par ? ip1 : ip0;
+  const bool active = i0 < ip;
+  const int nd0 = nd_first + NB * cgl;
+  const int ntile = d.nr_p >> 4;
+  const int nchf = d.nchf;
+  const int sgl = (lane >> 4) & 1;  // this lane's slot (column l%32 / 16)
+  const int lk = lane & 15;         // and radius within the tile
+
+  const int gsl[2] = {(L + M) * nchf * 2 * d.ncomp, (L - M) * nchf * 2 * d.ncomp};
+  const int erow0 = (int)tb.eoff[2 * M], erow1 = (int)tb.eoff[2 * M + 1];
+  const unsigned fsrc = (unsigned)frag_src(lane);
+  const unsigned lbase = lds_addr(smem);
+  auto issue = [&](int c, int b) {
+    const unsigned bufa = lbase + 16u * (unsigned)(b * LY::I4);
+#pragma unroll
+    for (int j = 0; j < LY::KB; j++) {
+      const int b = wave + W * j;
+      const int bb = b < LY::NBB ? b : 0;  // spare copies repeat block 0
+      const int sgb = bb / (2 * NB * S), h = (bb / (NB * S)) & 1, f = bb / S % NB, s = bb % S;
+      const int gb = gsl[sgb] + (c * 2 + h) * d.ncomp + nd0 + f;
+      copy_block(dig + (size_t)((gb * S + s) * ntile + rt) * 1024, fsrc, bufa + 1024u * b);
+    }
+    const unsigned exa = bufa + 16u * LY::EX;
+#pragma unroll
+    for (int j = 0; j < LY::KE; j++) {
+      const int task = wave + W * j;
+      const int *base = tb.eP;  // spare copies
+      unsigned off = 0;
+      if (task < NB) {
+        const int g = task * 4 + (lane >> 4);
+        const int sgb = g / (2 * NB), h = (g / NB) & 1, f = g % NB;
+        const int gb = gsl[sgb] + (c * 2 + h) * d.ncomp + nd0 + f;
+        base = dexp;
+        off = 4u * (unsigned)(gb * d.nr_p + rt * 16 + (lane & 15));
+      } else if (task < NB + (VEC ? 6 : 2)) {
+        const int tw = task - NB, tab = tw >> 1, pr = tw & 1;
+        const int ipp = pr ? ip1 : ip0;
+        const int row = ib * 64 + lane;
+        base = tab == 0 ? tb.eP : tab == 1 ? tb.edP : tb.ePa;
+        off = row < ipp ? 4u * (unsigned)((pr ? erow1 : erow0) + c * ipp + row) : 0u;
+      }
+      copy_dword(base, off, exa + 256u * task);
+    }
+  };
+
+  double C[NO][16];
+#pragma unroll
+  for (int o = 0; o < NO; o++)
+#pragma unroll
+    for (int e = 0; e < 16; e++) C[o][e] = 0.0;
+
+  const int hx = par, hy = 1 - par;
+  // A rows: the wave's 32 rows, clamped into the (M, parity) table (its
+  // rows are padded to 16 only); idle waves load from row 0 of the table so
+  // that every wave issues the same vector-memory operations
+  const int arow = active ? (int)(tb.off[2 * M + par] >> 6) + i0 : 0;
+  const int ipa = active ? ip : 0;
+  const int rmax = active ? ip - 1 - i0 : 31;
+  const int arl = (lane & 31) < rmax ? (lane & 31) : rmax;
+  const unsigned asrc0 = (unsigned)(arl * 64 + 16 * (lane >> 5));
+  auto load_a = [&](const signed char *tab, int c, i4 (&a)[S][2]) {
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const signed char *rowp = tab + (size_t)(arow + (c * S + s) * ipa) * 64;
+      a[s][0] = load_slice_ws(rowp, asrc0);
+      a[s][1] = load_slice_ws(rowp, asrc0 + 32u);
+    }
+  };
+  auto landed2 = [&](i4 (&a)[S][2]) {
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      asm volatile("" : "+v"(a[s][0]));
+      asm volatile("" : "+v"(a[s][1]));
+    }
+  };
+  constexpr int KCOPY = LY::KB + LY::KE;
+  i4 aP[S][2], adP[S][2], aPa[S][2];
+  load_a(tb.P, 0, aP);
+  if (VEC) load_a(tb.dP, 0, adP);
+  issue(0, 0);
+  if (VEC) load_a(tb.Pa, 0, aPa);
+  // this lane's 16-byte offset inside a 1 KiB data block for K half kh
+  const int boff0 = lk + 16 * (lane >> 5);
+  const int sgc0 = nsig == 1 ? 0 : 1;  // conjugate of slot 0 (slot 1's is 0)
+  const double msl = sgl == 0 ? (double)M : -(double)M;
+  for (int c = 0; c < nchf; c++) {
+    wait_vm<VEC ? 2 * S : 0>();
+    __syncthreads();
+    const i4 *buf = smem + (c % NBUF) * LY::I4;
+    const int *ex = reinterpret_cast<const int *>(buf + LY::EX);
+    const int cn = c + 1 < nchf ? c + 1 : c;
+    if (NBUF == 2) issue(cn, (c + 1) % NBUF);
+    {
+#define QSLOT(q) ((VEC && ((q) == 2 || (q) == 4)) ? (sgl == 0 ? sgc0 : 0) : sgl)
+#define QH(q) ((!VEC || (q) == 0 || (q) == 2 || (q) == 4) ? hx : hy)
+#define QF(q) (!VEC ? (q) : (q) == 0 ? 0 : ((q) == 1 || (q) == 4) ? 1 : 2)
+#define BOF(q, t, kh) buf[(((QSLOT(q) * 2 + QH(q)) * NB + QF(q)) * S + (t)) * 64 + boff0 + 32 * (kh)]
+#define KSHT_FWD32_QUANTITY(Q, TAB)                                                        \
+  {                                                                                        \
+    constexpr int qq = (Q);                                                                \
+    double T[16];                                                                          \
+    KSHT_SLICED32(qq, T)                                                                   \
+    constexpr int o = VEC ? (qq == 0 ? 0 : (qq <= 2) ? 1 : 2) : qq;                        \
+    const double cf = (!VEC || qq <= 1) ? 1.0 : (qq == 3 ? -1.0 : -msl);                   \
+    const int eb = ex[((QSLOT(qq) * 2 + QH(qq)) * NB + QF(qq)) * 16 + lk];                  \
+    const int *er = ex + 64 * NB + (TAB) * 128 + par * 64 + tile * 32 + 4 * (lane >> 5);    \
+    _Pragma("unroll") for (int e4 = 0; e4 < 4; e4++) {                                     \
+      const i4 ea = *reinterpret_cast<const i4 *>(er + 8 * e4);                            \
+      _Pragma("unroll") for (int r = 0; r < 4; r++) {                                      \
+        const int e = 4 * e4 + r;                                                          \
+        C[o][e] = fma(ldexp(T[e], ea[r] + eb - 20), cf, C[o][e]);                          \
+      }                                                                                    \
+    }                                                                                      \
+  }
+#define AOF(s, kh) AA_[s][kh]
+      landed2(aP);
+#define AA_ aP
+      if (active) {
+        KSHT_FWD32_QUANTITY(0, 0)
+        if (!VEC) {
+          if (NB > 1) KSHT_FWD32_QUANTITY(1 % NB, 0)
+          if (NB > 2) KSHT_FWD32_QUANTITY(2 % NB, 0)
+        }
+      }
+#undef AA_
+      load_a(tb.P, cn, aP);
+      if (VEC) {
+        landed2(adP);
+#define AA_ adP
+        if (active) {
+          KSHT_FWD32_QUANTITY(1, 1)
+          KSHT_FWD32_QUANTITY(3, 1)
+        }
+#undef AA_
+        load_a(tb.dP, cn, adP);
+        wait_vm<(NBUF == 2 ? KCOPY : 0) + 4 * S>();
+        landed2(aPa);
+#define AA_ aPa
+        if (active) {
+          KSHT_FWD32_QUANTITY(2, 2)
+          KSHT_FWD32_QUANTITY(4, 2)
+        }
+#undef AA_
+      }
+#undef KSHT_FWD32_QUANTITY
+#undef AOF
+#undef BOF
+#undef QSLOT
+#undef QH
+#undef QF
+    }
+    if (NBUF == 1 && c + 1 < nchf) {
+      __syncthreads();
+      issue(c + 1, 0);
+    }
+    if (VEC) load_a(tb.Pa, cn, aPa);
+  }
+  wait_vm<0>();
+  landed2(aP);
+  if (VEC) {
+    landed2(adP);
+    landed2(aPa);
+  }
+  if (!active) return;
+  const int k = rt * 16 + lk;
+  if (k >= d.nr || sgl >= nsig) return;
+  const double r1 = radius[k], r2 = r1 * r1;
+  const long long obase = (long long)jst[sgl == 0 ? L + M : L - M] * sa.s_j +
+                          (long long)k * sa.s_k + (long long)nd0 * sa.s_nd;
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    const int ii = i0 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+    const int l = M + 2 * ii + par;
+    if (l > L) continue;
+    const double g7 = g7s_of(l, M), g6 = g6s_of(l, M);
+    const long long o = obase + (long long)(l - M) * sa.s_j;
+    if (VEC) {
+      sp[o] = r2 * g7 * C[0][e];
+      sp[o + sa.s_nd] = r1 * g7 * C[1 % NO][e];
+      sp[o + 2 * sa.s_nd] = r1 * g7 * C[2 % NO][e];
+    } else {
+#pragma unroll
+      for (int q = 0; q < NO; q++) sp[o + (long long)q * sa.s_nd] = g6 * C[q][e];
+    }
+  }
+}
+#undef KSHT_SLICED32
+
+// ---------------------------------------------------------------------------
 // Host side.
 static int table_layouts(sht_plan *p, SplitState *st) {
   const int L = p->L, S = st->S;
@@ -1209,6 +1502,12 @@ static constexpr int fwd_lds(int nb) {
 }
 
 template <int S>
+static constexpr int fwd32_lds(int nb) {
+  return nb == 3 ? FwdLds32<S, 3>::bytes() : nb == 2 ? FwdLds32<S, 2>::bytes()
+                                                    : FwdLds32<S, 1>::bytes();
+}
+
+template <int S>
 static int slice_tables(sht_plan *p, SplitState *st) {
   const int L = p->L;
   const int nchb_max = bchunk_count(0, (L + 2) / 2);
@@ -1238,6 +1537,10 @@ static int slice_tables(sht_plan *p, SplitState *st) {
   KSHT_LDS((k_leg_fwd_split<S, 3, false>), fwd_lds<S>(3));
   KSHT_LDS((k_leg_fwd_split<S, 2, false>), fwd_lds<S>(2));
   KSHT_LDS((k_leg_fwd_split<S, 1, false>), fwd_lds<S>(1));
+  KSHT_LDS((k_leg_fwd_split32<S, 3, true>), fwd32_lds<S>(3));
+  KSHT_LDS((k_leg_fwd_split32<S, 3, false>), fwd32_lds<S>(3));
+  KSHT_LDS((k_leg_fwd_split32<S, 2, false>), fwd32_lds<S>(2));
+  KSHT_LDS((k_leg_fwd_split32<S, 1, false>), fwd32_lds<S>(1));
 #undef KSHT_LDS
   return SHT_OK;
 }
@@ -1375,11 +1678,18 @@ static int fwd_split_S(sht_plan *p, const Workspace &ws, int nv, int ns, const d
     SplitDims d = split_dims(p, ncomp, (ne_max + 63) / 64);
     d.ncg = g[i].ncg;
     const long long nblocks = (long long)(p->L + 1) * d.n_rt * d.ncg * d.nblk;
-    const dim3 grid((unsigned)nblocks), blk(kGemmThreads);
+    const dim3 grid((unsigned)nblocks);
+    // INT8 MFMA form of the products: 32x32x32 (4-wave work-groups) or
+    // 16x16x64 (8 waves), the plan's "split_fwd_mfma" variant
 #define KSHT_FWD(NB, VEC)                                                                      \
-  hipLaunchKernelGGL((k_leg_fwd_split<S, NB, VEC>), grid, blk, fwd_lds<S>(NB), p->stream, d, \
-                     g[i].nd_first, tb, ws.dig, ws.dexp, sp, sa, p->t.jst, p->t.ipad,        \
-                     p->t.radius)
+  if (p->var_fwd_mfma == 32)                                                                   \
+    hipLaunchKernelGGL((k_leg_fwd_split32<S, NB, VEC>), grid, dim3(kGemm32Threads),            \
+                       fwd32_lds<S>(NB), p->stream, d, g[i].nd_first, tb, ws.dig, ws.dexp, sp, \
+                       sa, p->t.jst, p->t.ipad, p->t.radius);                                  \
+  else                                                                                         \
+    hipLaunchKernelGGL((k_leg_fwd_split<S, NB, VEC>), grid, dim3(kGemmThreads),                \
+                       fwd_lds<S>(NB), p->stream, d, g[i].nd_first, tb, ws.dig, ws.dexp, sp,   \
+                       sa, p->t.jst, p->t.ipad, p->t.radius)
     if (g[i].vec) {
       StageTimer tk(p, 6);
       KSHT_FWD(3, true);

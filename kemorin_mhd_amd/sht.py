@@ -82,6 +82,8 @@ def lib() -> ctypes.CDLL:
         L.sht_plan_get_table.argtypes = [_vp, ctypes.c_char_p, _vp, ctypes.c_longlong]
         L.sht_plan_device_bytes.argtypes = [_vp]
         L.sht_plan_device_bytes.restype = ctypes.c_longlong
+        L.sht_plan_set_variant.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int]
+        L.sht_plan_get_variant.argtypes = [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
         for nm in ("sht_backward", "sht_forward", "sht_backward_host", "sht_forward_host",
                    "sht_legendre_backward", "sht_legendre_forward", "sht_backward_noalloc"):
             getattr(L, nm).argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]
@@ -129,6 +131,9 @@ def last_error() -> str:
 def _check(rc: int, where: str):
     if rc != SHT_OK:
         raise ShtError(rc, where)
+
+
+HIP_STREAM_LEGACY = 1  # hipStreamLegacy: the legacy default (null) stream
 
 
 def _torch_stream(*bufs):
@@ -237,14 +242,18 @@ class SphTransPlan:
 
     def _bind(self, *bufs):
         """Order a call on torch device tensors like a torch operation, unless
-        set_stream chose a stream: on torch's current stream, or for torch's
-        default stream (handle 0) on the plan's own stream, a blocking stream
-        that the default stream is ordered with."""
+        set_stream chose a stream: on torch's current stream, and for torch's
+        default stream (handle 0) on HIP's legacy default stream
+        (hipStreamLegacy), the same queue.  (A separate blocking stream is
+        not enough: an asynchronous host-to-device copy that torch has just
+        queued on the default stream was seen still in flight when a kernel
+        on such a stream started.)"""
         if self._user_stream:
             return
         s = _torch_stream(*bufs)
         if s is not None:
-            _check(lib().sht_plan_set_stream(self._h, s or None), "sht_plan_set_stream")
+            _check(lib().sht_plan_set_stream(self._h, s or HIP_STREAM_LEGACY),
+                   "sht_plan_set_stream")
 
     STAGES = ("legendre_backward", "fft_backward", "fft_forward", "legendre_forward", "exchange",
               "legendre_backward_vector_kernel", "legendre_forward_vector_kernel")
@@ -259,6 +268,17 @@ class SphTransPlan:
         calls = (ctypes.c_longlong * n)()
         _check(lib().sht_plan_timer_read(self._h, ms, calls, n), "sht_plan_timer_read")
         return {s: (ms[i], int(calls[i])) for i, s in enumerate(self.STAGES)}
+
+    def set_variant(self, name: str, value: int):
+        """Kernel variant of this plan (sht_plan_set_variant)."""
+        _check(lib().sht_plan_set_variant(self._h, name.encode(), int(value)),
+               "sht_plan_set_variant")
+
+    def get_variant(self, name: str) -> int:
+        v = ctypes.c_int()
+        _check(lib().sht_plan_get_variant(self._h, name.encode(), ctypes.byref(v)),
+               "sht_plan_get_variant")
+        return v.value
 
     def device_bytes(self) -> int:
         return int(lib().sht_plan_device_bytes(self._h))

@@ -102,3 +102,33 @@ def test_split_component_groups(case, torch_cuda, nv, ns):
         assert eb <= TOL[6] and ef <= TOL[6]
     finally:
         plan.close()
+
+
+@pytest.mark.parametrize("S", [4, 6, 7])
+@pytest.mark.parametrize("nv,ns", [(3, 0), (1, 2), (0, 4)])
+def test_forward_mfma32_bitexact_vs_mfma16(case, torch_cuda, S, nv, ns):
+    """The forward products on v_mfma_i32_32x32x32_i8 sum the same exact
+    INT32 digit products as the 16x16x64 form and fold them in the same
+    order, so both variants agree bit for bit; the 32x32 result is also held
+    to the oracle bar."""
+    name, cfg, orc = case
+    t = torch_cuda
+    nc = 3 * nv + ns
+    print(f"[gpu] split forward mfma32 {name} S={S} nv={nv} ns={ns}", flush=True)
+    plan = SphTransPlan(cfg, max_ncomp=nc, precision=SHT_PREC_SPLIT_INT8, n_slices=S)
+    try:
+        vr = np.random.default_rng(43).standard_normal(cfg.n_rtm(nc))
+        d_vr = t.from_numpy(vr).cuda()
+        out = {}
+        for v in (16, 32):
+            plan.set_variant("split_fwd_mfma", v)
+            assert plan.get_variant("split_fwd_mfma") == v
+            o = t.full((cfg.n_rj(nc),), np.nan, dtype=t.float64, device="cuda")
+            plan.legendre_forward(nv, ns, d_vr, o)
+            t.cuda.synchronize()
+            out[v] = o.cpu().numpy()
+        assert np.array_equal(out[16], out[32])
+        ref = orc.leg_fwd(nv, ns, vr)
+        assert rel_comp(out[32], ref, nc) <= TOL[S]
+    finally:
+        plan.close()

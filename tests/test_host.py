@@ -312,3 +312,19 @@ def test_wait_check_flags_missing_wait_states():
     br = ("\tv_readfirstlane_b32 s1, v7\n\ts_cbranch_scc1 .LBB0_1\n\ts_nop 7\n"
           ".LBB0_1:\n\tglobal_load_dwordx4 v[2:5], v1, s[0:1]\n\ts_waitcnt vmcnt(0)\n")
     assert any("SGPR" in p for p in _waits(br))
+
+
+def test_built_libraries_are_current():
+    """The shared libraries that travel with the tree (the HIP library and the
+    oracle) are newer than every source they are built from."""
+    csrc = os.path.join(ROOT, "kemorin_mhd_amd", "csrc")
+    lib = os.path.join(ROOT, "kemorin_mhd_amd", "libkemorin_sht.so")
+    srcs = [os.path.join(csrc, f) for f in os.listdir(csrc)] + [
+        os.path.join(ROOT, "include", "kemorin_sht.h")]
+    orc = os.path.join(ROOT, "oracle", "liboracle.so")
+    for so, deps in ((lib, srcs), (orc, [os.path.join(ROOT, "oracle", "sht_oracle.c"),
+                                         os.path.join(ROOT, "oracle", "sht_oracle.h")])):
+        if not os.path.exists(so):
+            pytest.skip(f"{so} not built")
+        stale = [d for d in deps if os.path.getmtime(d) > os.path.getmtime(so)]
+        assert not stale, f"{os.path.basename(so)} is older than {stale}"
