@@ -39,6 +39,13 @@ CONFIGS = {
     # round trip with the split plan does not fit in 288 GB with all four
     # field buffers resident)
     "cfg4_nr65": dict(L=1023, nr=65, nth=1536, nphi=3072),
+    # the large-truncation grid at all 257 radii: P_l^m generated per call one
+    # tile of orders at a time (the stored tables would be 12.9 GB, their
+    # split slices 12.1 GB more), and the backward writes its 8 grid
+    # components into the first 8 of the forward's 9 (so the forward of a
+    # step transforms the backward's output), which keeps the split plan and
+    # the field buffers within 288 GB
+    "cfg4": dict(L=1023, nr=257, nth=1536, nphi=3072, otf=True, alias=True),
 }
 METRIC = "SHT round trips/s + % roofline at L=511,Nr=257: split-prec vs FP64 MFMA"
 
@@ -206,7 +213,7 @@ def make_config(cfg_d):
     return ShellConfig(cfg_d["L"], cfg_d["nr"], cfg_d["nth"], cfg_d["nphi"])
 
 
-def make_inputs(cfg, torch, slope=1.5, seed=20261202):
+def make_inputs(cfg, torch, slope=1.5, seed=20261202, otf=False):
     """Seeded spectra sigma(l, r) = (l+1)^-slope sin(pi (r-ri)/(ro-ri)) for the
     backward (8 comps), and a band-limited grid for the forward (9 comps), made
     by the FP64 engine's backward of such spectra."""
@@ -228,7 +235,7 @@ def make_inputs(cfg, torch, slope=1.5, seed=20261202):
     sp_b = spectra(8)
     grid_f = torch.empty(cfg.n_rtp(9), dtype=torch.float64, device=dev)
     tmp = spectra(9)
-    plan = SphTransPlan(cfg, max_ncomp=9, precision=SHT_PREC_FP64)
+    plan = SphTransPlan(cfg, max_ncomp=9, precision=SHT_PREC_FP64, p_on_the_fly=otf)
     plan.sph_backward_transforms(3, 0, tmp, grid_f)
     torch.cuda.synchronize()
     plan.close()
@@ -236,19 +243,23 @@ def make_inputs(cfg, torch, slope=1.5, seed=20261202):
     return sp_b, grid_f
 
 
-def run_engine(cfg, precision, args, torch, sp_b, grid_f, keep=False):
+def run_engine(cfg, precision, args, torch, sp_b, grid_f, keep=False, otf=False, alias=False):
     from kemorin_mhd_amd import SHT_PREC_FP64, SHT_PREC_SPLIT_INT8, SphTransPlan
     prec = SHT_PREC_SPLIT_INT8 if precision == "split" else SHT_PREC_FP64
     stream = torch.cuda.Stream()
     torch.cuda.synchronize()
-    plan = SphTransPlan(cfg, max_ncomp=9, precision=prec, n_slices=args.slices)
+    plan = SphTransPlan(cfg, max_ncomp=9, precision=prec, n_slices=args.slices,
+                        p_on_the_fly=otf)
     plan.set_stream(stream.cuda_stream)
     apply_variants(plan, args)
     dev = "cuda"
     with torch.cuda.stream(stream):
-        grid_b = torch.empty(cfg.n_rtp(8), dtype=torch.float64, device=dev)
+        # aliased: the backward's 8 grid components are the forward's first 8
+        grid_b = (grid_f[: cfg.n_rtp(8)] if alias
+                  else torch.empty(cfg.n_rtp(8), dtype=torch.float64, device=dev))
         sp_f = torch.empty(cfg.n_rj(9), dtype=torch.float64, device=dev)
     stream.synchronize()
+    free_b, total_b = torch.cuda.mem_get_info()
 
     def step():
         plan.sph_backward_transforms(2, 2, sp_b, grid_b)
@@ -276,7 +287,8 @@ def run_engine(cfg, precision, args, torch, sp_b, grid_f, keep=False):
     torch.cuda.empty_cache()
     ms = (t1 - t0) * 1e3 / args.steps
     return dict(ms_per_step=ms, round_trips_per_s=1e3 / ms, timers=timers, plan_bytes=dev_bytes,
-                outputs=out, slices=(args.slices or 6) if precision == "split" else 0)
+                outputs=out, slices=(args.slices or 6) if precision == "split" else 0,
+                hbm_in_use_bytes=total_b - free_b, hbm_total_bytes=total_b)
 
 
 FIELDS_BWD = (("vector1", 0, 3), ("vector2", 3, 6), ("scalar1", 6, 7), ("scalar2", 7, 8))
@@ -475,11 +487,14 @@ def main():
     if split_ok and args.precision in ("auto", "both", "split"):
         precs.append("split")
     cfg = make_config(cfg_d)
-    sp_b, grid_f = make_inputs(cfg, torch)
+    otf, alias = cfg_d.get("otf", False), cfg_d.get("alias", False)
+    sp_b, grid_f = make_inputs(cfg, torch, otf=otf)
     results = {}
-    keep = "split" in precs and "fp64" in precs
+    # the split-vs-FP64 comparison keeps both engines' outputs (not with aliasing)
+    keep = "split" in precs and "fp64" in precs and not alias
     for p in precs:
-        results[p] = run_engine(cfg, p, args, torch, sp_b, grid_f, keep=keep)
+        results[p] = run_engine(cfg, p, args, torch, sp_b, grid_f, keep=keep, otf=otf,
+                                alias=alias)
         print(f"# {p}: {results[p]['ms_per_step']:.3f} ms/round trip", file=sys.stderr, flush=True)
     accuracy = None
     if keep:
@@ -502,13 +517,16 @@ def main():
                                 f"Ntheta={cfg_d['nth']} Nphi={cfg_d['nphi']}: backward 2 vector + "
                                 f"2 scalar (8 comps), forward 3 vector (9 comps)"),
                    "engine": head, "L": cfg_d["L"], "nr": cfg_d["nr"], "ntheta": cfg_d["nth"],
-                   "nphi": cfg_d["nphi"], "variants": args.variant},
+                   "nphi": cfg_d["nphi"], "variants": args.variant,
+                   "p_on_the_fly": otf, "grid_buffers_aliased": alias},
         "roofline": roof,
         "gflops_fp64_equivalent": F_rt / (res["ms_per_step"] / 1e3) / 1e9,
         "stages": per_stage,
         "engines": {p: {"ms_per_step": r["ms_per_step"], "round_trips_per_s": r["round_trips_per_s"],
                         "stage_ms_per_step": {s: v[0] / args.steps for s, v in r["timers"].items()},
-                        "plan_device_bytes": r["plan_bytes"]}
+                        "plan_device_bytes": r["plan_bytes"],
+                        "hbm_in_use_bytes": r["hbm_in_use_bytes"],
+                        "hbm_total_bytes": r["hbm_total_bytes"]}
                     for p, r in results.items()},
     }
     if "split" in results and "fp64" in results:

@@ -56,8 +56,18 @@ typedef struct {
   int max_ncomp;    /* largest ncomp a call will pass (workspace size)    */
   int precision;    /* SHT_PREC_FP64 or SHT_PREC_SPLIT_INT8              */
   int n_slices;     /* split mode: INT8 slices per operand (0 = default) */
-  int flags;        /* reserved, pass 0                                  */
+  int flags;        /* SHT_FLAG_* bits, 0 for none                      */
 } sht_desc;
+
+/* Descriptor flags.
+ * SHT_FLAG_P_ON_THE_FLY: do not store P_l^m and dP_l^m/dtheta (nor, in split
+ * mode, their INT8 slices) for all orders; every transform generates them
+ * in FP64 for one tile of orders m at a time into a tile buffer (about an
+ * eighth of the tables) just before that tile's Legendre products.  For
+ * large truncations (L = 1023: 12.9 GB of FP64 tables and 12.1 GB of
+ * slices), at the cost of the recurrence on every call.  Results are bit for
+ * bit those of stored tables. */
+#define SHT_FLAG_P_ON_THE_FLY 1
 
 /* Replaces initialize_sph_trans (SS/init_sph_trans.f90:46-74) and the
  * Legendre/FFT set-up it runs: Gauss points with the reference's Newton

@@ -134,6 +134,13 @@ struct sht_plan {
   long long device_bytes = 0;
   // split-precision state (allocated only in SHT_PREC_SPLIT_INT8 mode)
   void *split = nullptr;
+  // P tables generated per call, one m-tile at a time, into tile buffers
+  // (SHT_FLAG_P_ON_THE_FLY) instead of stored for all orders
+  int p_otf = 0;
+  std::vector<std::pair<int, int>> mtiles;          // [M0, M1) per tile
+  long long *offb_r = nullptr, *offf_r = nullptr;   // device: offsets rebased to the tile
+  double *Pt = nullptr, *dPt = nullptr;             // device: one tile of P and dP/dtheta
+  long long tile_elems = 0;
   // kernel variants (sht_plan_set_variant): INT8 MFMA form of the forward
   // split GEMM, 16 (16x16x64) or 32 (32x32x32)
   int var_fwd_mfma = 16;
@@ -202,6 +209,17 @@ int launch_gather_rev(sht_plan *p, int ncomp, long long n_dst, long long n_src,
                       const int *irev, const double *src, double *dst);
 // P table generation (sht_plan.hip)
 int generate_p_tables(sht_plan *p);
+// One m-tile of the FP64 P tables: orders [M0, M1), per-(M, parity) block
+// offsets (indexed by the absolute order) into the table pointers.  With
+// stored tables there is one tile of all orders; with tables per call each
+// tile is generated (in the layout of the direction) when prepared.
+struct PTile {
+  int M0, M1;
+  const long long *offb, *offf;
+  const double *Pb, *dPb, *Pf, *dPf;
+};
+int ptile_count(const sht_plan *p);
+int ptile_prepare(sht_plan *p, int i, bool fwd, PTile *t);
 int pole_tables_init(sht_plan *p);
 // split precision (sht_split.hip)
 int split_init(sht_plan *p);

@@ -39,6 +39,7 @@ __device__ __forceinline__ double g6_of(int l, int M) {
 
 struct LegDims {
   int L, nr, nth, nth_h, nth_hp, nv, ns, n_rt, ncg, nblk;
+  int M0, nM;  // orders M0 .. M0+nM-1 of this launch (one tile of the P tables)
 };
 
 // ---------------------------------------------------------------------------
@@ -71,7 +72,7 @@ __global__ void __launch_bounds__(256) k_leg_bwd(
   const int cgl = (int)(b % d.ncg);
   b /= d.ncg;
   const int rt = (int)(b % d.n_rt);
-  const int M = (int)(b / d.n_rt);
+  const int M = d.M0 + (int)(b / d.n_rt);
   const int L = d.L;
   const int n = L - M + 1, ne = (n + 1) / 2;
   const int nsig = (M == 0) ? 1 : 2;
@@ -280,8 +281,8 @@ __global__ void __launch_bounds__(256) k_leg_fwd(
   b /= d.ncg;
   const int rt = (int)(b % d.n_rt);
   b /= d.n_rt;
-  const int M = (int)(b % (d.L + 1));
-  const int ib = (int)(b / (d.L + 1));
+  const int M = d.M0 + (int)(b % d.nM);
+  const int ib = (int)(b / d.nM);
   const int L = d.L;
   const int ip0 = ipad[2 * M], ip1 = ipad[2 * M + 1];
   if (ib * 64 >= ip0) return;  // whole work-group beyond the degrees of M
@@ -450,29 +451,38 @@ static LegDims make_dims(const sht_plan *p, int nv, int ns, int nblk) {
   d.n_rt = p->nr_p / 16;
   d.ncg = nv + (ns + kSclPer - 1) / kSclPer;
   d.nblk = nblk;
+  d.M0 = 0;
+  d.nM = p->L + 1;
   return d;
 }
 
 int launch_leg_bwd(sht_plan *p, int nv, int ns, const double *sp, SpecAddr sa, double *out,
                    GridAddr ga) {
   const int nsg = (ns + kSclPer - 1) / kSclPer;
-  for (int vec = 1; vec >= 0; vec--) {
-    const int ncg = vec ? nv : nsg;
-    if (ncg == 0) continue;
-    LegDims d = make_dims(p, nv, ns, p->nth_hp / 64);
-    d.ncg = ncg;
-    const long long nblocks = (long long)(p->L + 1) * d.n_rt * d.ncg * d.nblk;
-    const int cg0 = vec ? 0 : nv;
-    if (vec) {
-      StageTimer tk(p, 5);
-      hipLaunchKernelGGL(k_leg_bwd<true>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d, cg0,
-                         sp, sa, out, ga, p->t.jst, p->t.offb, p->t.ipad, p->t.Pb, p->t.dPb,
-                         p->t.a_r, p->t.asin_t);
-    } else
-      hipLaunchKernelGGL(k_leg_bwd<false>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d,
-                         cg0, sp, sa, out, ga, p->t.jst, p->t.offb, p->t.ipad, p->t.Pb, p->t.dPb,
-                         p->t.a_r, p->t.asin_t);
-    KSHT_HIP(hipGetLastError());
+  for (int ti = 0; ti < ptile_count(p); ti++) {
+    PTile pt;
+    int rc = ptile_prepare(p, ti, false, &pt);
+    if (rc) return rc;
+    for (int vec = 1; vec >= 0; vec--) {
+      const int ncg = vec ? nv : nsg;
+      if (ncg == 0) continue;
+      LegDims d = make_dims(p, nv, ns, p->nth_hp / 64);
+      d.ncg = ncg;
+      d.M0 = pt.M0;
+      d.nM = pt.M1 - pt.M0;
+      const long long nblocks = (long long)d.nM * d.n_rt * d.ncg * d.nblk;
+      const int cg0 = vec ? 0 : nv;
+      if (vec) {
+        StageTimer tk(p, 5);
+        hipLaunchKernelGGL(k_leg_bwd<true>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d,
+                           cg0, sp, sa, out, ga, p->t.jst, pt.offb, p->t.ipad, pt.Pb, pt.dPb,
+                           p->t.a_r, p->t.asin_t);
+      } else
+        hipLaunchKernelGGL(k_leg_bwd<false>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d,
+                           cg0, sp, sa, out, ga, p->t.jst, pt.offb, p->t.ipad, pt.Pb, pt.dPb,
+                           p->t.a_r, p->t.asin_t);
+      KSHT_HIP(hipGetLastError());
+    }
   }
   return SHT_OK;
 }
@@ -481,23 +491,30 @@ int launch_leg_fwd(sht_plan *p, int nv, int ns, const double *in, GridAddr ga, d
                    SpecAddr sa) {
   const int ne_max = (p->L + 2) / 2;
   const int nsg = (ns + kSclPer - 1) / kSclPer;
-  for (int vec = 1; vec >= 0; vec--) {
-    const int ncg = vec ? nv : nsg;
-    if (ncg == 0) continue;
-    LegDims d = make_dims(p, nv, ns, (ne_max + 63) / 64);
-    d.ncg = ncg;
-    const long long nblocks = (long long)(p->L + 1) * d.n_rt * d.ncg * d.nblk;
-    const int cg0 = vec ? 0 : nv;
-    if (vec) {
-      StageTimer tk(p, 6);
-      hipLaunchKernelGGL(k_leg_fwd<true>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d, cg0,
-                         in, ga, sp, sa, p->t.jst, p->t.offf, p->t.ipad, p->t.Pf, p->t.dPf,
-                         p->t.radius, p->t.weight, p->t.asin_t);
-    } else
-      hipLaunchKernelGGL(k_leg_fwd<false>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d,
-                         cg0, in, ga, sp, sa, p->t.jst, p->t.offf, p->t.ipad, p->t.Pf, p->t.dPf,
-                         p->t.radius, p->t.weight, p->t.asin_t);
-    KSHT_HIP(hipGetLastError());
+  for (int ti = 0; ti < ptile_count(p); ti++) {
+    PTile pt;
+    int rc = ptile_prepare(p, ti, true, &pt);
+    if (rc) return rc;
+    for (int vec = 1; vec >= 0; vec--) {
+      const int ncg = vec ? nv : nsg;
+      if (ncg == 0) continue;
+      LegDims d = make_dims(p, nv, ns, (ne_max + 63) / 64);
+      d.ncg = ncg;
+      d.M0 = pt.M0;
+      d.nM = pt.M1 - pt.M0;
+      const long long nblocks = (long long)d.nM * d.n_rt * d.ncg * d.nblk;
+      const int cg0 = vec ? 0 : nv;
+      if (vec) {
+        StageTimer tk(p, 6);
+        hipLaunchKernelGGL(k_leg_fwd<true>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d,
+                           cg0, in, ga, sp, sa, p->t.jst, pt.offf, p->t.ipad, pt.Pf, pt.dPf,
+                           p->t.radius, p->t.weight, p->t.asin_t);
+      } else
+        hipLaunchKernelGGL(k_leg_fwd<false>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d,
+                           cg0, in, ga, sp, sa, p->t.jst, pt.offf, p->t.ipad, pt.Pf, pt.dPf,
+                           p->t.radius, p->t.weight, p->t.asin_t);
+      KSHT_HIP(hipGetLastError());
+    }
   }
   return SHT_OK;
 }

@@ -6,6 +6,7 @@
 // const_gauss_points_coefs (MHD/Fortran_src/CORE_src/gauss_integration/
 // gauss_integration.f90:60-133), schmidt_legendres_m
 // (MHD/Fortran_src/CORE_src/spherical_harmonics/schmidt_fix_m.f90:62-175).
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -164,6 +165,108 @@ __global__ void __launch_bounds__(256) k_gen_p(
   }
 }
 
+// One thread per (hemisphere row t < nth_hp, order M0 + blockIdx.y): one
+// tile of one layout (fwd: theta-major, i fastest; else i-major, theta
+// fastest), every entry of the tile's blocks written, padding included, so
+// the tile buffer needs no clearing between tiles (and no memset enters a
+// captured graph).  Same recurrence and operation order as k_gen_p.
+__global__ void __launch_bounds__(256) k_gen_p_tile(
+    int L, int M0, int nth_h, int nth_hp, int fwd, const double *__restrict__ cos_t,
+    const double *__restrict__ sin_t, const long long *__restrict__ off,
+    const int *__restrict__ ipad, double *__restrict__ P, double *__restrict__ dP) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int M = M0 + blockIdx.y;
+  if (t >= nth_hp) return;
+  const int ipp[2] = {ipad[2 * M], ipad[2 * M + 1]};
+  auto at = [&](int par, int i) -> long long {
+    return fwd ? off[2 * M + par] + (long long)t * ipp[par] + i
+               : off[2 * M + par] + (long long)i * nth_hp + t;
+  };
+  const int n = L - M + 1;
+  const int np[2] = {(n + 1) / 2, n / 2};
+  if (t >= nth_h) {  // padded rows
+    for (int par = 0; par < 2; par++)
+      for (int i = 0; i < ipp[par]; i++) {
+        P[at(par, i)] = 0.0;
+        dP[at(par, i)] = 0.0;
+      }
+    return;
+  }
+  const double c = cos_t[t], s = sin_t[t];
+  SchmidtStream rm, rn, rp;
+  rm.init(M, c, s);
+  if (M >= 1) {
+    rn.init(M - 1, c, s);
+    rn.advance();
+  }
+  const bool have_p = (M + 1 <= L);
+  if (have_p) rp.init(M + 1, c, s);
+  for (int l = M; l <= L; l++) {
+    const double pv = rm.value();
+    double dp;
+    if (M == 0) {
+      dp = (l == 0) ? 0.0 : -sqrt((double)(l * (l + 1) / 2)) * rp.value();
+    } else if (M == 1) {
+      dp = (l == 1) ? rn.value()
+                    : 0.5 * (sqrt((double)(2 * l * (l + 1))) * rn.value() -
+                             sqrt((double)((l - 1) * (l + 2))) * rp.value());
+    } else {
+      dp = (l == M) ? 0.5 * sqrt((double)(2 * M)) * rn.value()
+                    : 0.5 * (sqrt((double)((l + M) * (l - M + 1))) * rn.value() -
+                             sqrt((double)((l - M) * (l + M + 1))) * rp.value());
+    }
+    const long long o = at((l - M) & 1, (l - M) >> 1);
+    P[o] = pv;
+    dP[o] = dp;
+    rm.advance();
+    if (M >= 1) rn.advance();
+    if (have_p && l >= M + 1) rp.advance();
+  }
+  for (int par = 0; par < 2; par++)
+    for (int i = np[par]; i < ipp[par]; i++) {
+      P[at(par, i)] = 0.0;
+      dP[at(par, i)] = 0.0;
+    }
+}
+
+// Orders [M0, M1) per tile: about an eighth of the table elements each.
+static void partition_mtiles(sht_plan *p) {
+  const int L = p->L;
+  long long total = 0;
+  for (int M = 0; M <= L; M++)
+    total += (long long)(p->h_ipad[2 * M] + p->h_ipad[2 * M + 1]) * p->nth_hp;
+  const long long target = (total + 7) / 8;
+  p->mtiles.clear();
+  int M0 = 0;
+  long long acc = 0;
+  for (int M = 0; M <= L; M++) {
+    acc += (long long)(p->h_ipad[2 * M] + p->h_ipad[2 * M + 1]) * p->nth_hp;
+    if (acc >= target || M == L) {
+      p->mtiles.push_back({M0, M + 1});
+      M0 = M + 1;
+      acc = 0;
+    }
+  }
+}
+
+int ptile_count(const sht_plan *p) { return p->p_otf ? (int)p->mtiles.size() : 1; }
+
+int ptile_prepare(sht_plan *p, int i, bool fwd, PTile *t) {
+  if (!p->p_otf) {
+    *t = PTile{0, p->L + 1, p->t.offb, p->t.offf, p->t.Pb, p->t.dPb, p->t.Pf, p->t.dPf};
+    return SHT_OK;
+  }
+  const int M0 = p->mtiles[i].first, M1 = p->mtiles[i].second;
+  const long long *off = fwd ? p->offf_r : p->offb_r;
+  hipLaunchKernelGGL(k_gen_p_tile, dim3((p->nth_hp + 255) / 256, M1 - M0), dim3(256), 0,
+                     p->stream, p->L, M0, p->nth_h, p->nth_hp, fwd ? 1 : 0, p->t.cos_t,
+                     p->t.sin_t, off, p->t.ipad, p->Pt, p->dPt);
+  KSHT_HIP(hipGetLastError());
+  *t = PTile{M0, M1, p->offb_r, p->offf_r, p->Pt, p->dPt, p->Pt, p->dPt};
+  return SHT_OK;
+}
+
 int generate_p_tables(sht_plan *p) {
   const int L = p->L;
   p->h_offb.assign(2 * (L + 1), 0);
@@ -185,12 +288,41 @@ int generate_p_tables(sht_plan *p) {
   p->p_table_elems_b = ob;
   p->p_table_elems_f = of;
   const size_t bb = sizeof(double) * (size_t)ob, bf = sizeof(double) * (size_t)of;
+  if (p->p_otf) {
+    // offsets rebased to each tile's first block, tile buffers for the largest tile
+    partition_mtiles(p);
+    std::vector<long long> rb(2 * (L + 1)), rf(2 * (L + 1));
+    long long most = 0;
+    for (const auto &tl : p->mtiles) {
+      const long long b0 = p->h_offb[2 * tl.first], f0 = p->h_offf[2 * tl.first];
+      for (int q = 2 * tl.first; q < 2 * tl.second; q++) {
+        rb[q] = p->h_offb[q] - b0;
+        rf[q] = p->h_offf[q] - f0;
+      }
+      const long long end = tl.second <= L ? p->h_offb[2 * tl.second] : ob;
+      most = std::max(most, end - b0);
+    }
+    p->tile_elems = most;
+    KSHT_HIP(hipMalloc(&p->offb_r, sizeof(long long) * 2 * (L + 1)));
+    KSHT_HIP(hipMalloc(&p->offf_r, sizeof(long long) * 2 * (L + 1)));
+    KSHT_HIP(hipMemcpy(p->offb_r, rb.data(), sizeof(long long) * 2 * (L + 1), hipMemcpyHostToDevice));
+    KSHT_HIP(hipMemcpy(p->offf_r, rf.data(), sizeof(long long) * 2 * (L + 1), hipMemcpyHostToDevice));
+  }
   KSHT_HIP(hipMalloc(&p->t.offb, sizeof(long long) * 2 * (L + 1)));
   KSHT_HIP(hipMalloc(&p->t.offf, sizeof(long long) * 2 * (L + 1)));
   KSHT_HIP(hipMalloc(&p->t.ipad, sizeof(int) * 2 * (L + 1)));
   KSHT_HIP(hipMemcpy(p->t.offb, p->h_offb.data(), sizeof(long long) * 2 * (L + 1), hipMemcpyHostToDevice));
   KSHT_HIP(hipMemcpy(p->t.offf, p->h_offf.data(), sizeof(long long) * 2 * (L + 1), hipMemcpyHostToDevice));
   KSHT_HIP(hipMemcpy(p->t.ipad, p->h_ipad.data(), sizeof(int) * 2 * (L + 1), hipMemcpyHostToDevice));
+  if (p->p_otf) {
+    const size_t tb = sizeof(double) * (size_t)p->tile_elems;
+    if (hipMalloc(&p->Pt, tb) != hipSuccess || hipMalloc(&p->dPt, tb) != hipSuccess) {
+      set_error("P tiles: device allocation of %zu bytes failed", 2 * tb);
+      return SHT_ERR_OOM;
+    }
+    p->device_bytes += 2 * (long long)tb;
+    return SHT_OK;
+  }
   if (hipMalloc(&p->t.Pb, bb) != hipSuccess || hipMalloc(&p->t.dPb, bb) != hipSuccess ||
       hipMalloc(&p->t.Pf, bf) != hipSuccess || hipMalloc(&p->t.dPf, bf) != hipSuccess) {
     set_error("P tables: device allocation of %zu bytes failed", 2 * (bb + bf));
@@ -226,7 +358,7 @@ static void free_plan(sht_plan *p) {
   void *ptrs[] = {p->t.offb, p->t.offf, p->t.ipad, p->t.jst, p->t.Pb, p->t.dPb,
                   p->t.Pf, p->t.dPf, p->t.radius, p->t.a_r, p->t.weight, p->t.asin_t,
                   p->t.cos_t, p->t.sin_t, p->t.tw, p->t.twn, p->t.slot_off_X, p->t.slot_off_rtm,
-                  p->t.pole, p->X};
+                  p->t.pole, p->X, p->offb_r, p->offf_r, p->Pt, p->dPt};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   split_free(p);
@@ -263,7 +395,9 @@ extern "C" int sht_plan_create(const sht_desc *d, sht_plan **out) {
   if (d->precision == SHT_PREC_SPLIT_INT8 && d->n_slices != 0 &&
       (d->n_slices < 4 || d->n_slices > 7))
     BAD("n_slices = %d: need 0 (default) or 4..7 in split mode", d->n_slices);
-  if (d->flags != 0) BAD("flags = %d: reserved, pass 0", d->flags);
+  if (d->flags & ~SHT_FLAG_P_ON_THE_FLY)
+    BAD("flags = %d: unknown bits (SHT_FLAG_P_ON_THE_FLY = %d is the only flag)", d->flags,
+        SHT_FLAG_P_ON_THE_FLY);
 #undef BAD
   for (int k = 0; k < nr; k++)
     if (!(d->radius[k] > 0.0)) {
@@ -283,6 +417,7 @@ extern "C" int sht_plan_create(const sht_desc *d, sht_plan **out) {
   p->max_ncomp = d->max_ncomp;
   p->precision = d->precision;
   p->n_slices = d->n_slices;
+  p->p_otf = (d->flags & SHT_FLAG_P_ON_THE_FLY) ? 1 : 0;
   p->own_stream = nullptr;
   int rc = SHT_OK;
 #define CHECK(x)          \

@@ -72,6 +72,10 @@ struct SplitState {
   int nchf = 0;                // forward chunks of 64 hemisphere rows
   signed char *dig = nullptr;  // digits, fragment order
   int *dexp = nullptr;         // per (chunk, component, radius) exponents
+  // P computed per call (plan flag): slices of one m-tile at a time, at
+  // offsets rebased to the tile's first block; the tile buffers above (bP ..
+  // fePa) then hold one tile, the backward and forward sharing them
+  long long *b_off_r = nullptr, *b_eoff_r = nullptr, *f_off_r = nullptr, *f_eoff_r = nullptr;
 };
 
 __device__ __forceinline__ i4 mfma8(i4 a, i4 b, i4 c) {
@@ -128,11 +132,11 @@ __device__ __forceinline__ unsigned pack4(const long long *Fb, int k) {
 // bytes.  Backward: value (row t, K i) = Pb[offb + i*nth_hp + t].  Forward:
 // value (row i, K t) = Pf[offf + t*ipad + i] (* 1/sin(theta_t) for Pa).
 template <int S>
-__global__ void __launch_bounds__(256) k_slice_b(int L, int nth_hp, const long long *offb,
+__global__ void __launch_bounds__(256) k_slice_b(int L, int M0, int nth_hp, const long long *offb,
                                                  const int *ipad, const double *P,
                                                  const long long *soff, const long long *eoff,
                                                  signed char *out, int *eout) {
-  const int M = blockIdx.z >> 1, par = blockIdx.z & 1;
+  const int M = M0 + (blockIdx.z >> 1), par = blockIdx.z & 1;
   const int n = L - M + 1, np = par ? n / 2 : (n + 1) / 2;
   const int nch = bchunk_count(M, np);
   const int c = blockIdx.y;
@@ -167,12 +171,12 @@ __global__ void __launch_bounds__(256) k_slice_b(int L, int nth_hp, const long l
 }
 
 template <int S>
-__global__ void __launch_bounds__(256) k_slice_f(int L, int nth_h, int nth_hp, const long long *offf,
+__global__ void __launch_bounds__(256) k_slice_f(int L, int M0, int nth_h, int nth_hp, const long long *offf,
                                                  const int *ipad, const double *P,
                                                  const double *asin_t, int use_asin,
                                                  const long long *soff, const long long *eoff,
                                                  signed char *out, int *eout) {
-  const int M = blockIdx.z >> 1, par = blockIdx.z & 1;
+  const int M = M0 + (blockIdx.z >> 1), par = blockIdx.z & 1;
   const int ip = ipad[2 * M + par];
   const int c = blockIdx.y;  // theta chunk
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -558,6 +562,7 @@ __device__ __forceinline__ void landed(i4 (&a)[S]) {
 
 struct SplitDims {
   int L, nr, nr_p, nth, nth_h, nth_hp, ncomp, n_rt, ncg, nblk, nchf;
+  int M0, nM;  // orders M0 .. M0+nM-1 of this launch (one tile of the P slices)
 };
 
 // Relative scale of group pair (g0, g0+1) (or of the single last group g0)
@@ -663,7 +668,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
   const int cgl = (int)(bidx % d.ncg);
   bidx /= d.ncg;
   const int rt = (int)(bidx % d.n_rt);
-  const int M = (int)(bidx / d.n_rt);
+  const int M = d.M0 + (int)(bidx / d.n_rt);
   const int L = d.L;
   const int n = L - M + 1, np0 = (n + 1) / 2, np1 = n / 2;
   const int nch0 = bchunk_count(M, np0), nch1 = bchunk_count(M, np1);
@@ -932,8 +937,8 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_fwd_split(
   bidx /= d.ncg;
   const int rt = (int)(bidx % d.n_rt);
   bidx /= d.n_rt;
-  const int M = (int)(bidx % (d.L + 1));
-  const int ib = (int)(bidx / (d.L + 1));
+  const int M = d.M0 + (int)(bidx % d.nM);
+  const int ib = (int)(bidx / d.nM);
   const int L = d.L;
   const int ip0 = ipad[2 * M], ip1 = ipad[2 * M + 1];
   if (ib * 64 >= ip0) return;  // whole work-group beyond the degrees of M
@@ -1223,8 +1228,8 @@ k_leg_fwd_split32(
   bidx /= d.ncg;
   const int rt = (int)(bidx % d.n_rt);
   bidx /= d.n_rt;
-  const int M = (int)(bidx % (d.L + 1));
-  const int ib = (int)(bidx / (d.L + 1));
+  const int M = d.M0 + (int)(bidx % d.nM);
+  const int ib = (int)(bidx / d.nM);
   const int L = d.L;
   const int ip0 = ipad[2 * M], ip1 = ipad[2 * M + 1];
   if (ib * 64 >= ip0) return;  // whole work-group beyond the degrees of M
@@ -1472,13 +1477,53 @@ static int table_layouts(sht_plan *p, SplitState *st) {
   };
   int rc = SHT_OK;
   const size_t nb = 2 * (L + 1) * sizeof(long long);
+  if (p->p_otf) {
+    // one tile's slices, both directions in the same buffers; offsets rebased
+    std::vector<long long> rbo(2 * (L + 1)), rbe(2 * (L + 1)), rfo(2 * (L + 1)), rfe(2 * (L + 1));
+    long long mb = 0, mbe = 0, mf = 0, mfe = 0;
+    for (const auto &tl : p->mtiles) {
+      const int q0 = 2 * tl.first, q1 = 2 * tl.second;
+      for (int q = q0; q < q1; q++) {
+        rbo[q] = st->hb_off[q] - st->hb_off[q0];
+        rbe[q] = st->hb_eoff[q] - st->hb_eoff[q0];
+        rfo[q] = st->hf_off[q] - st->hf_off[q0];
+        rfe[q] = st->hf_eoff[q] - st->hf_eoff[q0];
+      }
+      const bool last = q1 >= 2 * (L + 1);
+      mb = std::max(mb, (last ? bo : st->hb_off[q1]) - st->hb_off[q0]);
+      mbe = std::max(mbe, (last ? be : st->hb_eoff[q1]) - st->hb_eoff[q0]);
+      mf = std::max(mf, (last ? fo : st->hf_off[q1]) - st->hf_off[q0]);
+      mfe = std::max(mfe, (last ? fe : st->hf_eoff[q1]) - st->hf_eoff[q0]);
+    }
+    if ((rc = alloc((void **)&st->b_off_r, nb)) || (rc = alloc((void **)&st->b_eoff_r, nb)) ||
+        (rc = alloc((void **)&st->f_off_r, nb)) || (rc = alloc((void **)&st->f_eoff_r, nb)))
+      return rc;
+    KSHT_HIP(hipMemcpy(st->b_off_r, rbo.data(), nb, hipMemcpyHostToDevice));
+    KSHT_HIP(hipMemcpy(st->b_eoff_r, rbe.data(), nb, hipMemcpyHostToDevice));
+    KSHT_HIP(hipMemcpy(st->f_off_r, rfo.data(), nb, hipMemcpyHostToDevice));
+    KSHT_HIP(hipMemcpy(st->f_eoff_r, rfe.data(), nb, hipMemcpyHostToDevice));
+    const long long so = std::max(mb, mf), se = std::max(mbe, mfe);
+    // the backward uses bP, bdP (beP, bedP), the forward fP, fdP, fPa: three
+    // buffers serve both
+    if ((rc = alloc((void **)&st->fP, so)) || (rc = alloc((void **)&st->fdP, so)) ||
+        (rc = alloc((void **)&st->fPa, so)) || (rc = alloc((void **)&st->feP, se * 4)) ||
+        (rc = alloc((void **)&st->fedP, se * 4)) || (rc = alloc((void **)&st->fePa, se * 4)))
+      return rc;
+    st->bP = st->fP;
+    st->bdP = st->fdP;
+    st->beP = st->feP;
+    st->bedP = st->fedP;
+  }
   if ((rc = alloc((void **)&st->b_off, nb)) || (rc = alloc((void **)&st->b_eoff, nb)) ||
       (rc = alloc((void **)&st->f_off, nb)) || (rc = alloc((void **)&st->f_eoff, nb)) ||
-      (rc = alloc((void **)&st->bP, bo)) || (rc = alloc((void **)&st->bdP, bo)) ||
-      (rc = alloc((void **)&st->beP, be * 4)) || (rc = alloc((void **)&st->bedP, be * 4)) ||
-      (rc = alloc((void **)&st->fP, fo)) || (rc = alloc((void **)&st->fdP, fo)) ||
-      (rc = alloc((void **)&st->fPa, fo)) || (rc = alloc((void **)&st->feP, fe * 4)) ||
-      (rc = alloc((void **)&st->fedP, fe * 4)) || (rc = alloc((void **)&st->fePa, fe * 4)) ||
+      (!p->p_otf && ((rc = alloc((void **)&st->bP, bo)) || (rc = alloc((void **)&st->bdP, bo)) ||
+                     (rc = alloc((void **)&st->beP, be * 4)) ||
+                     (rc = alloc((void **)&st->bedP, be * 4)) ||
+                     (rc = alloc((void **)&st->fP, fo)) || (rc = alloc((void **)&st->fdP, fo)) ||
+                     (rc = alloc((void **)&st->fPa, fo)) ||
+                     (rc = alloc((void **)&st->feP, fe * 4)) ||
+                     (rc = alloc((void **)&st->fedP, fe * 4)) ||
+                     (rc = alloc((void **)&st->fePa, fe * 4)))) ||
       (rc = alloc((void **)&st->cb, st->hcb.size() * sizeof(long long))) ||
       (rc = alloc((void **)&st->dig, (size_t)(groups * S * ntile * 1024))) ||
       (rc = alloc((void **)&st->dexp, (size_t)(groups * p->nr_p * 4))))
@@ -1507,25 +1552,68 @@ static constexpr int fwd32_lds(int nb) {
                                                     : FwdLds32<S, 1>::bytes();
 }
 
+// Slices of the P tables of orders [M0, M1) for one direction, from the FP64
+// tables of tile pt into the slice tables at offsets (off, eoff).
+template <int S>
+static int slice_range(sht_plan *p, SplitState *st, const PTile &pt, bool fwd,
+                       const long long *off, const long long *eoff) {
+  const int L = p->L, M0 = pt.M0, nM = pt.M1 - pt.M0;
+  if (!fwd) {
+    const int nchb_max = bchunk_count(M0, (L - M0 + 2) / 2);
+    const dim3 gb((p->nth_hp + 255) / 256, nchb_max, 2 * nM);
+    hipLaunchKernelGGL(k_slice_b<S>, gb, dim3(256), 0, p->stream, L, M0, p->nth_hp, pt.offb,
+                       p->t.ipad, pt.Pb, off, eoff, st->bP, st->beP);
+    hipLaunchKernelGGL(k_slice_b<S>, gb, dim3(256), 0, p->stream, L, M0, p->nth_hp, pt.offb,
+                       p->t.ipad, pt.dPb, off, eoff, st->bdP, st->bedP);
+  } else {
+    int ipmax = 0;
+    for (int q = 2 * pt.M0; q < 2 * pt.M1; q++) ipmax = std::max(ipmax, p->h_ipad[q]);
+    const dim3 gf((ipmax + 255) / 256, p->nth_hp / 64, 2 * nM);
+    hipLaunchKernelGGL(k_slice_f<S>, gf, dim3(256), 0, p->stream, L, M0, p->nth_h, p->nth_hp,
+                       pt.offf, p->t.ipad, pt.Pf, p->t.asin_t, 0, off, eoff, st->fP, st->feP);
+    hipLaunchKernelGGL(k_slice_f<S>, gf, dim3(256), 0, p->stream, L, M0, p->nth_h, p->nth_hp,
+                       pt.offf, p->t.ipad, pt.dPf, p->t.asin_t, 0, off, eoff, st->fdP, st->fedP);
+    hipLaunchKernelGGL(k_slice_f<S>, gf, dim3(256), 0, p->stream, L, M0, p->nth_h, p->nth_hp,
+                       pt.offf, p->t.ipad, pt.Pf, p->t.asin_t, 1, off, eoff, st->fPa, st->fePa);
+  }
+  KSHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+
+// Tile ti of the P slices for one direction: with stored tables the whole
+// sliced table; with P per call, the FP64 tile generated and sliced.
+template <int S>
+static int split_tile(sht_plan *p, SplitState *st, int ti, bool fwd, SplitTabs *tb, int *M0,
+                      int *nM) {
+  PTile pt;
+  int rc = ptile_prepare(p, ti, fwd, &pt);
+  if (rc) return rc;
+  *M0 = pt.M0;
+  *nM = pt.M1 - pt.M0;
+  if (p->p_otf) {
+    rc = fwd ? slice_range<S>(p, st, pt, true, st->f_off_r, st->f_eoff_r)
+             : slice_range<S>(p, st, pt, false, st->b_off_r, st->b_eoff_r);
+    if (rc) return rc;
+  }
+  if (fwd)
+    *tb = SplitTabs{p->p_otf ? st->f_off_r : st->f_off, p->p_otf ? st->f_eoff_r : st->f_eoff,
+                    st->fP, st->fdP, st->fPa, st->feP, st->fedP, st->fePa};
+  else
+    *tb = SplitTabs{p->p_otf ? st->b_off_r : st->b_off, p->p_otf ? st->b_eoff_r : st->b_eoff,
+                    st->bP, st->bdP, nullptr, st->beP, st->bedP, nullptr};
+  return SHT_OK;
+}
+
 template <int S>
 static int slice_tables(sht_plan *p, SplitState *st) {
-  const int L = p->L;
-  const int nchb_max = bchunk_count(0, (L + 2) / 2);
-  const dim3 gb((p->nth_hp + 255) / 256, nchb_max, 2 * (L + 1));
-  hipLaunchKernelGGL(k_slice_b<S>, gb, dim3(256), 0, p->stream, L, p->nth_hp, p->t.offb,
-                     p->t.ipad, p->t.Pb, st->b_off, st->b_eoff, st->bP, st->beP);
-  hipLaunchKernelGGL(k_slice_b<S>, gb, dim3(256), 0, p->stream, L, p->nth_hp, p->t.offb,
-                     p->t.ipad, p->t.dPb, st->b_off, st->b_eoff, st->bdP, st->bedP);
-  int ipmax = 0;
-  for (int v : p->h_ipad) ipmax = v > ipmax ? v : ipmax;
-  const dim3 gf((ipmax + 255) / 256, p->nth_hp / 64, 2 * (L + 1));
-  hipLaunchKernelGGL(k_slice_f<S>, gf, dim3(256), 0, p->stream, L, p->nth_h, p->nth_hp, p->t.offf,
-                     p->t.ipad, p->t.Pf, p->t.asin_t, 0, st->f_off, st->f_eoff, st->fP, st->feP);
-  hipLaunchKernelGGL(k_slice_f<S>, gf, dim3(256), 0, p->stream, L, p->nth_h, p->nth_hp, p->t.offf,
-                     p->t.ipad, p->t.dPf, p->t.asin_t, 0, st->f_off, st->f_eoff, st->fdP, st->fedP);
-  hipLaunchKernelGGL(k_slice_f<S>, gf, dim3(256), 0, p->stream, L, p->nth_h, p->nth_hp, p->t.offf,
-                     p->t.ipad, p->t.Pf, p->t.asin_t, 1, st->f_off, st->f_eoff, st->fPa, st->fePa);
-  KSHT_HIP(hipGetLastError());
+  if (!p->p_otf) {
+    PTile pt;
+    int rc = ptile_prepare(p, 0, false, &pt);
+    if (rc) return rc;
+    if ((rc = slice_range<S>(p, st, pt, false, st->b_off, st->b_eoff)) ||
+        (rc = slice_range<S>(p, st, pt, true, st->f_off, st->f_eoff)))
+      return rc;
+  }
   // LDS limits of the GEMM kernels (dynamic shared memory above 64 KiB)
 #define KSHT_LDS(K, BYTES) \
   KSHT_HIP(hipFuncSetAttribute((const void *)(K), hipFuncAttributeMaxDynamicSharedMemorySize, (BYTES)))
@@ -1569,9 +1657,14 @@ int split_init(sht_plan *p) {
 void split_free(sht_plan *p) {
   SplitState *st = (SplitState *)p->split;
   if (!st) return;
+  if (p->p_otf) {  // the backward's tile buffers are the forward's
+    st->bP = st->bdP = nullptr;
+    st->beP = st->bedP = nullptr;
+  }
   void *ptrs[] = {st->b_off, st->b_eoff, st->f_off, st->f_eoff, st->bP,  st->bdP,
                   st->beP,   st->bedP,   st->fP,    st->fdP,    st->fPa, st->feP,
-                  st->fedP,  st->fePa,   st->cb,    st->dig,    st->dexp};
+                  st->fedP,  st->fePa,   st->cb,    st->dig,    st->dexp, st->b_off_r,
+                  st->b_eoff_r, st->f_off_r, st->f_eoff_r};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   delete st;
@@ -1605,6 +1698,8 @@ static SplitDims split_dims(const sht_plan *p, int ncomp, int nblk) {
   d.ncg = 0;
   d.nblk = nblk;
   d.nchf = p->nth_hp / 64;
+  d.M0 = 0;
+  d.nM = p->L + 1;
   return d;
 }
 
@@ -1634,29 +1729,36 @@ static int bwd_split_S(sht_plan *p, const Workspace &ws, int nv, int ns, const d
                      p->L, p->nr, p->nr_p, nrad, nv, ncomp, spec, sa, p->t.jst, st->cb, ws.dig,
                      ws.dexp);
   KSHT_HIP(hipGetLastError());
-  SplitTabs tb{st->b_off, st->b_eoff, st->bP, st->bdP, nullptr, st->beP, st->bedP, nullptr};
   GroupLaunch g[3];
   const int ng = component_groups(nv, ns, g);
-  for (int i = 0; i < ng; i++) {
-    SplitDims d = split_dims(p, ncomp, p->nth_hp / 64);
-    d.ncg = g[i].ncg;
-    const long long nblocks = (long long)(p->L + 1) * d.n_rt * d.ncg * d.nblk;
-    const dim3 grid((unsigned)nblocks), blk(kGemmThreads);
+  for (int ti = 0; ti < ptile_count(p); ti++) {
+    SplitTabs tb;
+    int M0 = 0, nM = 0;
+    int rc = split_tile<S>(p, st, ti, false, &tb, &M0, &nM);
+    if (rc) return rc;
+    for (int i = 0; i < ng; i++) {
+      SplitDims d = split_dims(p, ncomp, p->nth_hp / 64);
+      d.ncg = g[i].ncg;
+      d.M0 = M0;
+      d.nM = nM;
+      const long long nblocks = (long long)nM * d.n_rt * d.ncg * d.nblk;
+      const dim3 grid((unsigned)nblocks), blk(kGemmThreads);
 #define KSHT_BWD(NB, VEC)                                                                      \
   hipLaunchKernelGGL((k_leg_bwd_split<S, NB, VEC>), grid, blk, bwd_lds<S>(NB), p->stream, d, \
                      g[i].nd_first, tb, st->cb, ws.dig, ws.dexp, out, ga, p->t.a_r,          \
                      p->t.asin_t)
-    if (g[i].vec) {
-      StageTimer tk(p, 5);
-      KSHT_BWD(3, true);
-    } else if (g[i].nb == 3)
-      KSHT_BWD(3, false);
-    else if (g[i].nb == 2)
-      KSHT_BWD(2, false);
-    else
-      KSHT_BWD(1, false);
+      if (g[i].vec) {
+        StageTimer tk(p, 5);
+        KSHT_BWD(3, true);
+      } else if (g[i].nb == 3)
+        KSHT_BWD(3, false);
+      else if (g[i].nb == 2)
+        KSHT_BWD(2, false);
+      else
+        KSHT_BWD(1, false);
 #undef KSHT_BWD
-    KSHT_HIP(hipGetLastError());
+      KSHT_HIP(hipGetLastError());
+    }
   }
   return SHT_OK;
 }
@@ -1670,17 +1772,23 @@ static int fwd_split_S(sht_plan *p, const Workspace &ws, int nv, int ns, const d
                      p->stream, p->nr, p->nr_p, p->nth, p->nth_h, st->nchf, ncomp, in, ga,
                      p->t.weight, ws.dig, ws.dexp);
   KSHT_HIP(hipGetLastError());
-  SplitTabs tb{st->f_off, st->f_eoff, st->fP, st->fdP, st->fPa, st->feP, st->fedP, st->fePa};
   const int ne_max = (p->L + 2) / 2;
   GroupLaunch g[3];
   const int ng = component_groups(nv, ns, g);
-  for (int i = 0; i < ng; i++) {
-    SplitDims d = split_dims(p, ncomp, (ne_max + 63) / 64);
-    d.ncg = g[i].ncg;
-    const long long nblocks = (long long)(p->L + 1) * d.n_rt * d.ncg * d.nblk;
-    const dim3 grid((unsigned)nblocks);
-    // INT8 MFMA form of the products: 32x32x32 (4-wave work-groups) or
-    // 16x16x64 (8 waves), the plan's "split_fwd_mfma" variant
+  for (int ti = 0; ti < ptile_count(p); ti++) {
+    SplitTabs tb;
+    int M0 = 0, nM = 0;
+    int rc = split_tile<S>(p, st, ti, true, &tb, &M0, &nM);
+    if (rc) return rc;
+    for (int i = 0; i < ng; i++) {
+      SplitDims d = split_dims(p, ncomp, (ne_max + 63) / 64);
+      d.ncg = g[i].ncg;
+      d.M0 = M0;
+      d.nM = nM;
+      const long long nblocks = (long long)nM * d.n_rt * d.ncg * d.nblk;
+      const dim3 grid((unsigned)nblocks);
+      // INT8 MFMA form of the products: 32x32x32 (4-wave work-groups) or
+      // 16x16x64 (8 waves), the plan's "split_fwd_mfma" variant
 #define KSHT_FWD(NB, VEC)                                                                      \
   if (p->var_fwd_mfma == 32)                                                                   \
     hipLaunchKernelGGL((k_leg_fwd_split32<S, NB, VEC>), grid, dim3(kGemm32Threads),            \
@@ -1690,17 +1798,18 @@ static int fwd_split_S(sht_plan *p, const Workspace &ws, int nv, int ns, const d
     hipLaunchKernelGGL((k_leg_fwd_split<S, NB, VEC>), grid, dim3(kGemmThreads),                \
                        fwd_lds<S>(NB), p->stream, d, g[i].nd_first, tb, ws.dig, ws.dexp, sp,   \
                        sa, p->t.jst, p->t.ipad, p->t.radius)
-    if (g[i].vec) {
-      StageTimer tk(p, 6);
-      KSHT_FWD(3, true);
-    } else if (g[i].nb == 3)
-      KSHT_FWD(3, false);
-    else if (g[i].nb == 2)
-      KSHT_FWD(2, false);
-    else
-      KSHT_FWD(1, false);
+      if (g[i].vec) {
+        StageTimer tk(p, 6);
+        KSHT_FWD(3, true);
+      } else if (g[i].nb == 3)
+        KSHT_FWD(3, false);
+      else if (g[i].nb == 2)
+        KSHT_FWD(2, false);
+      else
+        KSHT_FWD(1, false);
 #undef KSHT_FWD
-    KSHT_HIP(hipGetLastError());
+      KSHT_HIP(hipGetLastError());
+    }
   }
   return SHT_OK;
 }

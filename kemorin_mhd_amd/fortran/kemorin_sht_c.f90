@@ -14,6 +14,8 @@
       integer(c_int), parameter :: SHT_OK = 0
       integer(c_int), parameter :: SHT_PREC_FP64 = 0
       integer(c_int), parameter :: SHT_PREC_SPLIT_INT8 = 1
+!     descriptor flags: P_l^m generated per call, one tile of orders at a time
+      integer(c_int), parameter :: SHT_FLAG_P_ON_THE_FLY = 1
 !
 !>      Plan descriptor (sht_desc)
       type, bind(C) :: sht_desc
