@@ -65,6 +65,9 @@ def parse():
     ap.add_argument("--shells", type=int, default=0,
                     help="only run the batched-shell workload: this many seeded uneven shells, "
                          "one HIP graph launch against a loop of per-shell calls")
+    ap.add_argument("--autotune", metavar="CACHE", default=None,
+                    help="tune every plan's kernel variants (sht_plan_autotune) with this JSON "
+                         "cache before timing, and report the tuned against the default")
     ap.add_argument("--variant", action="append", default=[], metavar="NAME=VALUE",
                     help="kernel variant for every plan (sht_plan_set_variant), repeatable")
     ap.add_argument("--nonlinear", action="store_true",
@@ -148,13 +151,15 @@ def run_shells(args, torch):
            "fp64_equivalent_flop_per_step": flops, "engines": {}}
     stream = torch.cuda.Stream()
     for prec in precs:
-        plans, bufs = [], []
+        plans, bufs, tunes = [], [], []
         with torch.cuda.stream(stream):
             for c in shells:
                 p = SphTransPlan(c, max_ncomp=4, precision=SHT_PREC_SPLIT_INT8 if prec == "split"
                                  else SHT_PREC_FP64, n_slices=args.slices)
                 p.set_stream(stream.cuda_stream)
                 apply_variants(p, args)
+                if args.autotune:
+                    tunes.append(p.autotune(1, 1, 1, 1, cache=args.autotune))
                 ws = torch.randn(c.n_rj(4), generator=gen, dtype=torch.float64, device="cuda")
                 vin = torch.randn(c.n_rtp(4), generator=gen, dtype=torch.float64, device="cuda")
                 vout = torch.empty(c.n_rtp(4), dtype=torch.float64, device="cuda")
@@ -185,6 +190,12 @@ def run_shells(args, torch):
                          "gflops_fp64_equivalent": flops / ms / 1e6}
         res["graph_speedup_vs_loop"] = res["loop"]["ms_per_step"] / res["graph"]["ms_per_step"]
         res["plan_device_bytes"] = plan_bytes
+        if tunes:
+            # the tuner's own round-trip times of each shell, tuned against default
+            res["autotune"] = {
+                "tuned_ms_sum": sum(x["ms"] for x in tunes),
+                "default_ms_sum": sum(x["default_ms"] for x in tunes),
+                "chosen": [x["variants"] for x in tunes]}
         res["buffer_bytes"] = sum(8 * sum(t.numel() for t in b) for b in bufs)
         out["engines"][prec] = res
         batch.close()
@@ -252,6 +263,7 @@ def run_engine(cfg, precision, args, torch, sp_b, grid_f, keep=False, otf=False,
                         p_on_the_fly=otf)
     plan.set_stream(stream.cuda_stream)
     apply_variants(plan, args)
+    tuned = plan.autotune(2, 2, 3, 0, cache=args.autotune) if args.autotune else None
     dev = "cuda"
     with torch.cuda.stream(stream):
         # aliased: the backward's 8 grid components are the forward's first 8
@@ -288,7 +300,7 @@ def run_engine(cfg, precision, args, torch, sp_b, grid_f, keep=False, otf=False,
     ms = (t1 - t0) * 1e3 / args.steps
     return dict(ms_per_step=ms, round_trips_per_s=1e3 / ms, timers=timers, plan_bytes=dev_bytes,
                 outputs=out, slices=(args.slices or 6) if precision == "split" else 0,
-                hbm_in_use_bytes=total_b - free_b, hbm_total_bytes=total_b)
+                hbm_in_use_bytes=total_b - free_b, hbm_total_bytes=total_b, autotune=tuned)
 
 
 FIELDS_BWD = (("vector1", 0, 3), ("vector2", 3, 6), ("scalar1", 6, 7), ("scalar2", 7, 8))
@@ -326,7 +338,8 @@ def split_accuracy(cfg, args, torch, results):
         p["outputs"] = None
     torch.cuda.empty_cache()
     sp3, gf3 = make_inputs(cfg, torch, slope=3.0, seed=7)
-    one = argparse.Namespace(steps=1, warmup=0, slices=args.slices)
+    one = argparse.Namespace(steps=1, warmup=0, slices=args.slices, variant=args.variant,
+                             autotune=args.autotune)
     o64 = run_engine(cfg, "fp64", one, torch, sp3, gf3, keep=True)["outputs"]
     osp = run_engine(cfg, "split", one, torch, sp3, gf3, keep=True)["outputs"]
     acc["slope_3"] = field_errors(cfg, torch, osp, o64)
@@ -526,7 +539,8 @@ def main():
                         "stage_ms_per_step": {s: v[0] / args.steps for s, v in r["timers"].items()},
                         "plan_device_bytes": r["plan_bytes"],
                         "hbm_in_use_bytes": r["hbm_in_use_bytes"],
-                        "hbm_total_bytes": r["hbm_total_bytes"]}
+                        "hbm_total_bytes": r["hbm_total_bytes"],
+                        "autotune": r["autotune"]}
                     for p, r in results.items()},
     }
     if "split" in results and "fp64" in results:

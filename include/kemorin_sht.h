@@ -91,11 +91,29 @@ long long sht_plan_device_bytes(const sht_plan *plan);
  * MS/ = MHD/Fortran_src/MHD_src/sph_MHD/).  Names and values:
  *   "split_fwd_mfma": 16 or 32, the INT8 MFMA form of the forward split
  *                     Legendre products (v_mfma_i32_16x16x64_i8 or
- *                     v_mfma_i32_32x32x32_i8).
+ *                     v_mfma_i32_32x32x32_i8);
+ *   "fft_form":       1 or 0, the split-form FFT (Nphi = r0 2^k, 2^k > L;
+ *                     default where the length has it) or the mixed-radix
+ *                     Stockham FFT.
  * Results do not depend on the variant.  SHT_ERR_ARG for an unknown name or
  * value. */
 int sht_plan_set_variant(sht_plan *plan, const char *name, int value);
 int sht_plan_get_variant(const sht_plan *plan, const char *name, int *value);
+/* Tile-shape autotuner: times every applicable combination of the plan's
+ * variants ("split_fwd_mfma" for split-precision plans, "fft_form" where the
+ * length has the split form) on synthetic device data of the given field
+ * sets (one backward of nvector_b/nscalar_b and one forward of
+ * nvector_f/nscalar_f per round trip, best of three) and keeps the fastest.
+ * Replaces select_legendre_transform (MS/init_sphrical_transform_MHD.f90:
+ * 201-345), which times the reference's Legendre loop variants at start-up.
+ * cache_path (NULL = none): a JSON file of earlier results, one entry per
+ * shape (L, Nr, Ntheta, Nphi, precision, slices, flags, field sets); a hit
+ * sets the cached winner without timing, a miss appends the new winner.
+ * ms_out (NULL = none): [round-trip ms of the winner, of the default].
+ * Allocates temporary device buffers of the field sets' size; runs on the
+ * plan's stream and returns after it drained. */
+int sht_plan_autotune(sht_plan *plan, int nvector_b, int nscalar_b, int nvector_f,
+                      int nscalar_f, const char *cache_path, double *ms_out);
 
 /* Stage timers, the counterpart of the reference's elapsed-time ids for the
  * transform (MHD/Fortran_src/MHD_src/IO/init_sph_MHD_elapsed_label.f90:59-65:

@@ -1138,7 +1138,7 @@ int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, doub
         hipLaunchKernelGGL((K<kFftElemsLarge, false>), grid, dim3(kFftThreads), lds, p->stream, __VA_ARGS__); \
     }                                                                                      \
   } while (0)
-  if (p->fft.fast) {
+  if (p->fft.fast && p->var_fft_form) {
 #define KSHT_B(NP) KSHT_FFT3_CASE(k_fft3_bwd, NP, kFftNCb, p->fft.lds3b, p->nphi, p->fft.r0,   \
                                   p->fft.r0m, p->L, nline, X, xa, v, p->t.twn)
     switch (p->fft.np) {
@@ -1162,7 +1162,7 @@ int launch_fft_fwd(sht_plan *p, int ncomp, const double *v, double *X, FourierAd
   const int B = p->fft.lines_per_wg;
   const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);
   const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.line_elems;
-  if (p->fft.fast) {
+  if (p->fft.fast && p->var_fft_form) {
 #define KSHT_F(NP) KSHT_FFT3_CASE(k_fft3_fwd, NP, fft_ncf<NP>(), p->fft.lds3, p->nphi, p->fft.r0, \
                                   p->L, nline, v, X, xa, p->t.twn)
     switch (p->fft.np) {

@@ -144,6 +144,9 @@ struct sht_plan {
   // kernel variants (sht_plan_set_variant): INT8 MFMA form of the forward
   // split GEMM, 16 (16x16x64) or 32 (32x32x32)
   int var_fwd_mfma = 16;
+  // FFT form: 1 = split form n = r0 * 2^k where the plan has it, 0 = the
+  // mixed-radix Stockham form for every length
+  int var_fft_form = 1;
   // stage timers (sht_plan_timer_*): event pairs recorded per launch
   int timing = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> tev[ksht::kNumStages];

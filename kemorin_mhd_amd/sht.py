@@ -84,6 +84,9 @@ def lib() -> ctypes.CDLL:
         L.sht_plan_device_bytes.restype = ctypes.c_longlong
         L.sht_plan_set_variant.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int]
         L.sht_plan_get_variant.argtypes = [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
+        L.sht_plan_autotune.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_char_p,
+                                        ctypes.POINTER(ctypes.c_double)]
         for nm in ("sht_backward", "sht_forward", "sht_backward_host", "sht_forward_host",
                    "sht_legendre_backward", "sht_legendre_forward", "sht_backward_noalloc"):
             getattr(L, nm).argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]
@@ -280,6 +283,20 @@ class SphTransPlan:
         _check(lib().sht_plan_get_variant(self._h, name.encode(), ctypes.byref(v)),
                "sht_plan_get_variant")
         return v.value
+
+    VARIANTS = ("split_fwd_mfma", "fft_form")
+
+    def autotune(self, nvector_b: int, nscalar_b: int, nvector_f: int, nscalar_f: int,
+                 cache: str | None = None) -> dict:
+        """Time the plan's kernel variants for these field sets and keep the
+        fastest (sht_plan_autotune); returns the chosen variants and the
+        round-trip ms of the winner and of the default."""
+        ms = (ctypes.c_double * 2)()
+        _check(lib().sht_plan_autotune(self._h, nvector_b, nscalar_b, nvector_f, nscalar_f,
+                                       cache.encode() if cache else None, ms),
+               "sht_plan_autotune")
+        return {"variants": {v: self.get_variant(v) for v in self.VARIANTS},
+                "ms": ms[0], "default_ms": ms[1]}
 
     def device_bytes(self) -> int:
         return int(lib().sht_plan_device_bytes(self._h))

@@ -86,6 +86,15 @@ def test_plan_create_rejects_null_and_nonpositive_radius():
     assert "radius" in sht.last_error()
 
 
+def test_variants_and_autotune_reject_null_plan():
+    L = K.lib()
+    v = ctypes.c_int()
+    assert L.sht_plan_set_variant(None, b"split_fwd_mfma", 32) == 1
+    assert L.sht_plan_get_variant(None, b"fft_form", ctypes.byref(v)) == 1
+    assert L.sht_plan_autotune(None, 2, 2, 3, 0, None, None) == 1
+    assert "null" in K.last_error()
+
+
 def test_calls_on_null_plan_fail_cleanly():
     L = sht.lib()
     assert L.sht_backward(None, 1, 0, None, None) != 0
