@@ -92,9 +92,12 @@ long long sht_plan_device_bytes(const sht_plan *plan);
  *   "split_fwd_mfma": 16 or 32, the INT8 MFMA form of the forward split
  *                     Legendre products (v_mfma_i32_16x16x64_i8 or
  *                     v_mfma_i32_32x32x32_i8);
- *   "fft_form":       1 or 0, the split-form FFT (Nphi = r0 2^k, 2^k > L;
- *                     default where the length has it) or the mixed-radix
- *                     Stockham FFT.
+ *   "fft_form":       1, 0 or 2: the split-form FFT (Nphi = r0 2^k, 2^k > L;
+ *                     default where the length has it), the mixed-radix
+ *                     Stockham FFT, or the pruned real DFT as one FP64 GEMM
+ *                     per direction against a cos/sin basis (its bases and
+ *                     GEMM workspace are set up, and counted in the plan's
+ *                     device bytes, when it is first selected).
  * Results do not depend on the variant.  SHT_ERR_ARG for an unknown name or
  * value. */
 int sht_plan_set_variant(sht_plan *plan, const char *name, int value);

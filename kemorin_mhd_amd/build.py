@@ -15,7 +15,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkemorin_sht.so")
 SOURCES = ["sht_plan.hip", "sht_legendre.hip", "sht_fft.hip", "sht_reorder.hip",
            "sht_split.hip", "sht_capi.hip", "sht_batch.hip", "sht_pole.hip",
-           "sht_nonlinear.hip", "sht_copy.hip", "sht_coriolis.hip", "sht_tune.hip"]
+           "sht_nonlinear.hip", "sht_copy.hip", "sht_coriolis.hip", "sht_tune.hip", "sht_dft.hip"]
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-fno-gpu-rdc",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -81,7 +81,8 @@ def build(verbose: bool = False, incremental: bool = False) -> str:
         objs = list(ex.map(one, SOURCES))
         fut.result()
     tmp = LIB + ".tmp"
-    r = subprocess.run([cc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs],
+    r = subprocess.run([cc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs,
+                        "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"],
                        capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

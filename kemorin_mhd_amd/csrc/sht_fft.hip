@@ -1119,7 +1119,15 @@ int fft_plan_init(sht_plan *p) {
     hipLaunchKernelGGL((K<NP, nc_>), g3, dim3(NP / 8 * nc_), (LDS), p->stream, __VA_ARGS__); \
   } break;
 
+// the DFT form takes the plan's own Fourier workspace layout (rows of Nr
+// Ntheta contiguous lines); the reference's rtp layout keeps the FFT kernels
+static bool dft_applies(const sht_plan *p, const FourierAddr &xa) {
+  return p->var_fft_form == 2 && p->dft && !xa.fftpack && xa.s_line == 1 &&
+         xa.q_stride == (long long)p->nr * p->nth;
+}
+
 int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, double *v) {
+  if (dft_applies(p, xa)) return launch_dft_bwd(p, ncomp, X, xa, v);
   const long long nline = (long long)p->nr * p->nth;
   const int B = p->fft.lines_per_wg;
   const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);
@@ -1158,6 +1166,7 @@ int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, doub
 }
 
 int launch_fft_fwd(sht_plan *p, int ncomp, const double *v, double *X, FourierAddr xa) {
+  if (dft_applies(p, xa)) return launch_dft_fwd(p, ncomp, v, X, xa);
   const long long nline = (long long)p->nr * p->nth;
   const int B = p->fft.lines_per_wg;
   const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);

@@ -145,8 +145,10 @@ struct sht_plan {
   // split GEMM, 16 (16x16x64) or 32 (32x32x32)
   int var_fwd_mfma = 16;
   // FFT form: 1 = split form n = r0 * 2^k where the plan has it, 0 = the
-  // mixed-radix Stockham form for every length
+  // mixed-radix Stockham form for every length, 2 = the pruned real DFT as an
+  // FP64 GEMM (set-up on first selection: sht_dft.hip)
   int var_fft_form = 1;
+  void *dft = nullptr;
   // stage timers (sht_plan_timer_*): event pairs recorded per launch
   int timing = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> tev[ksht::kNumStages];
@@ -204,6 +206,11 @@ int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa,
                    double *v);
 int launch_fft_fwd(sht_plan *p, int ncomp, const double *v, double *X,
                    FourierAddr xa);
+// zonal transform as a DFT product (sht_dft.hip)
+int dft_init(sht_plan *p);
+void dft_free(sht_plan *p);
+int launch_dft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, double *v);
+int launch_dft_fwd(sht_plan *p, int ncomp, const double *v, double *X, FourierAddr xa);
 // reorders (sht_reorder.hip)
 int launch_zero_unused_rtp(sht_plan *p, int ncomp, double *rtp);
 int launch_rtm_to_rtp(sht_plan *p, int ncomp, const double *rtm, double *rtp);

@@ -362,6 +362,7 @@ static void free_plan(sht_plan *p) {
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   split_free(p);
+  dft_free(p);
   for (auto &v : p->tev)
     for (auto &e : v) {
       (void)hipEventDestroy(e.first);
@@ -521,7 +522,7 @@ extern "C" long long sht_plan_device_bytes(const sht_plan *p) { return p ? p->de
 
 // Kernel variants: name -> the plan field and its allowed values.
 static int *variant_slot(sht_plan *p, const char *name, const int **allowed, int *nallowed) {
-  static const int mfma[] = {16, 32}, form[] = {0, 1};
+  static const int mfma[] = {16, 32}, form[] = {0, 1, 2};
   if (!strcmp(name, "split_fwd_mfma")) {
     *allowed = mfma;
     *nallowed = 2;
@@ -529,7 +530,7 @@ static int *variant_slot(sht_plan *p, const char *name, const int **allowed, int
   }
   if (!strcmp(name, "fft_form")) {
     *allowed = form;
-    *nallowed = 2;
+    *nallowed = 3;
     return &p->var_fft_form;
   }
   return nullptr;
@@ -549,6 +550,11 @@ extern "C" int sht_plan_set_variant(sht_plan *p, const char *name, int value) {
   }
   for (int i = 0; i < n; i++)
     if (allowed[i] == value) {
+      // the DFT form sets up its bases and GEMM handle on first selection
+      if (slot == &p->var_fft_form && value == 2) {
+        const int rc = dft_init(p);
+        if (rc) return rc;
+      }
       *slot = value;
       return SHT_OK;
     }

@@ -7,8 +7,9 @@
 // fastest for the run.  Here the variants are the kernel forms of one plan:
 //   split_fwd_mfma  16 or 32: INT8 MFMA form of the forward split products
 //                   (split-precision plans);
-//   fft_form        1 or 0: split-form FFT (n = r0 2^k) or mixed-radix
-//                   Stockham FFT (plans whose length has the split form).
+//   fft_form        1, 0 or 2: split-form FFT (n = r0 2^k, where the length
+//                   has it), mixed-radix Stockham FFT, or the pruned real
+//                   DFT as an FP64 GEMM (sht_dft.hip).
 // Every combination runs one backward and one forward transform untimed,
 // then the best of three timed round trips (HIP events on the plan's
 // stream) decides.  Results are cached in a JSON file, one entry per line:
@@ -139,6 +140,7 @@ extern "C" int sht_plan_autotune(sht_plan *p, int nvb, int nsb, int nvf, int nsf
   std::vector<int> mfmas = {16}, forms = {p->fft.fast ? 1 : 0};
   if (p->precision == SHT_PREC_SPLIT_INT8) mfmas.push_back(32);
   if (p->fft.fast) forms.push_back(0);
+  if (dft_init(p) == SHT_OK) forms.push_back(2);  // the DFT product, where its set-up fits
   const int keep_mfma = p->var_fwd_mfma, keep_form = p->var_fft_form;
   const long long nnod = (long long)p->nr * p->nth * p->nphi;
   const long long nsp = (long long)p->jmax * p->nr;
