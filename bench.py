@@ -452,8 +452,9 @@ def cpu_baseline(cfg_d, nr_sample):
             "kind": "port",
             "sample": (f"L={cfg_d['L']} Ntheta={cfg_d['nth']} Nphi={cfg_d['nphi']}, {nr_s} of "
                        f"{cfg_d['nr']} radii, median of 3 round trips ({t:.2f} s each), "
-                       f"scaled linearly to {cfg_d['nr']} radii; oracle symmetric_BLAS_big + "
-                       f"FFTPACK-convention FFT, FP64, OpenMP")}
+                       f"scaled linearly to {cfg_d['nr']} radii; the oracle's restatement of "
+                       f"symmetric_BLAS_big with unblocked triple loops in place of DGEMM "
+                       f"(contraction off), FFTPACK-convention FFT, FP64, OpenMP over radii")}
 
 
 def main():
