@@ -92,19 +92,22 @@ long long sht_plan_device_bytes(const sht_plan *plan);
  *   "split_fwd_mfma": 16 or 32, the INT8 MFMA form of the forward split
  *                     Legendre products (v_mfma_i32_16x16x64_i8 or
  *                     v_mfma_i32_32x32x32_i8);
- *   "fft_form":       1, 0 or 2: the split-form FFT (Nphi = r0 2^k, 2^k > L;
- *                     default where the length has it), the mixed-radix
- *                     Stockham FFT, or the pruned real DFT as one FP64 GEMM
- *                     per direction against a cos/sin basis (its bases and
- *                     GEMM workspace are set up, and counted in the plan's
- *                     device bytes, when it is first selected).
- * Results do not depend on the variant.  SHT_ERR_ARG for an unknown name or
- * value. */
+ *   "fft_form":       1, 0, 2 or 3: the split-form FFT (Nphi = r0 2^k,
+ *                     2^k > L; default where the length has it), the
+ *                     mixed-radix Stockham FFT, the pruned real DFT as one
+ *                     FP64 GEMM per direction against a cos/sin basis, or
+ *                     the whole line as one chirp-z (Bluestein) convolution
+ *                     of power-of-two length MP >= Nphi + 2L (MP <= 2048; any
+ *                     prime factors).  Forms 2 and 3 set up their tables (and
+ *                     count them in the plan's device bytes) when first
+ *                     selected; SHT_ERR_UNSUPPORTED where they do not apply.
+ * Results do not depend on the variant beyond round-off.  SHT_ERR_ARG for an
+ * unknown name or value. */
 int sht_plan_set_variant(sht_plan *plan, const char *name, int value);
 int sht_plan_get_variant(const sht_plan *plan, const char *name, int *value);
 /* Tile-shape autotuner: times every applicable combination of the plan's
- * variants ("split_fwd_mfma" for split-precision plans, "fft_form" where the
- * length has the split form) on synthetic device data of the given field
+ * variants ("split_fwd_mfma" for split-precision plans, "fft_form" among the
+ * forms that apply to the length) on synthetic device data of the given field
  * sets (one backward of nvector_b/nscalar_b and one forward of
  * nvector_f/nscalar_f per round trip, best of three) and keeps the fastest.
  * Replaces select_legendre_transform (MS/init_sphrical_transform_MHD.f90:

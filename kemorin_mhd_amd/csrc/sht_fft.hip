@@ -479,6 +479,13 @@ __device__ __forceinline__ void pass_lds_order() {
     __syncthreads();
 }
 
+// exp(-2 pi i k / MP) for k < MP as hi[k / 64] lo[k % 64] (MP / 64 + 64
+// entries instead of MP): the chirp-z kernels keep their LDS for the lines
+struct TwoLevelRoots {
+  const double2 *hi, *lo;
+  __device__ __forceinline__ double2 operator[](int k) const { return cmul(hi[k >> 6], lo[k & 63]); }
+};
+
 // tw: roots of unity in LDS, of order NP at stride ts (the NP-entry table
 // with ts = 1, or the N-entry table with ts = R0)
 template <int NP, int Ns>
@@ -516,6 +523,49 @@ __device__ __forceinline__ void f3_passes(double2 (&v)[8], double2 *ln, int j,
       for (int r = 0; r < R2; r++) v[i * R2 + r] = ln[fpad(j + T * i + (NP / R2) * r)];
     pass_lds_order<NP>();
     f3_passes<NP, Ns * R>(v, ln, j, tw, ts, s);
+  }
+}
+
+// The same passes with the roots composed from two-level factors (order NP,
+// stride 1): for the chirp-z kernels, whose LDS holds the lines instead of
+// an NP-entry table.  A butterfly's twiddles w^r, r < R, are powers of its
+// one root w (one table lookup, R - 2 complex products, error growing as r).
+template <int NP, int Ns>
+__device__ __forceinline__ void f3_passes_2l(double2 (&v)[8], double2 *ln, int j,
+                                             const TwoLevelRoots &tw, double s) {
+  constexpr int T = NP / 8;
+  constexpr int R = (NP / Ns >= 8) ? 8 : NP / Ns;
+#pragma unroll
+  for (int i = 0; i < 8 / R; i++) {
+    const int b = j + T * i;
+    if constexpr (Ns > 1) {
+      const int k = b % Ns;
+      double2 w1 = tw[k * (NP / (Ns * R))];  // exp(-2 pi i k / (Ns R))
+      w1.y *= -s;
+      double2 w = w1;
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        v[i * R + r] = cmul(v[i * R + r], w);
+        if (r + 1 < R) w = cmul(w, w1);
+      }
+    }
+    dft_small<R>(v + i * R, s);
+  }
+  if constexpr (Ns * R < NP) {
+#pragma unroll
+    for (int i = 0; i < 8 / R; i++) {
+      const int b = j + T * i;
+#pragma unroll
+      for (int r = 0; r < R; r++) ln[fpad((b / Ns) * Ns * R + b % Ns + Ns * r)] = v[i * R + r];
+    }
+    pass_lds_order<NP>();
+    constexpr int R2 = (NP / (Ns * R) >= 8) ? 8 : NP / (Ns * R);
+#pragma unroll
+    for (int i = 0; i < 8 / R2; i++)
+#pragma unroll
+      for (int r = 0; r < R2; r++) v[i * R2 + r] = ln[fpad(j + T * i + (NP / R2) * r)];
+    pass_lds_order<NP>();
+    f3_passes_2l<NP, Ns * R>(v, ln, j, tw, s);
   }
 }
 
@@ -888,6 +938,442 @@ __global__ void __launch_bounds__(NC * NP / 8) k_fft3_fwd(int N, int R0, int L,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Chirp-z form (plan variant fft_form 3): the whole line as one Bluestein
+// convolution of length MP, a power of two >= N + 2L, so that any Nphi, with
+// any prime factors, costs two MP-point FFTs on the register / LDS passes of
+// the split form.  With w_k = exp(i pi k^2 / N) and 2 m j = m^2 + j^2 -
+// (j - m)^2:
+//   backward x_j = sum_{|m|<=L} Z_m e^{+2 pi i m j/N}
+//              = w_j sum_m (Z_m w_m) conj(w_{j-m}),            j < N,
+//   forward  Z_q = sum_{j<N} x_j e^{-2 pi i q j/N}
+//              = conj(w_q) sum_j (x_j conj(w_j)) w_{q-j},      |q| <= L.
+// Each sum is a linear convolution whose kernel spans N + 2L consecutive
+// indices (j - m in [-L, N-1+L], q - j in [-(N-1)-L, L]); with every index
+// taken mod MP >= N + 2L no two of them collide, so the cyclic convolution
+//   y = IDFT_MP(DFT_MP(u) . DFT_MP(h))
+// reproduces it at the wanted outputs.  DFT_MP(h) / MP is precomputed per
+// direction in long double.  Table (device): the roots exp(-2 pi i k / MP)
+// as MP/64 + 64 two-level factors, the backward and the forward filter (MP
+// entries each), then w_k, k < N.  The two real lines of a complex line pack
+// as in the other forms.
+//
+// A work-group owns NC complex lines (NC = 2, one at MP = 2048: measured
+// faster than 4 and 2, the work-groups being barrier-bound), T = MP/8 threads
+// per line, the root factors in LDS (the filters are read from L2, once per
+// element and line), and walks a range of line blocks (persistent).
+template <int MP>
+constexpr int czt_nc() { return MP >= 2048 ? 1 : 2; }
+template <int MP>
+constexpr int czt_hb() { return MP / 64 + 64; }  // table offsets: backward filter,
+template <int MP>
+constexpr int czt_hf() { return czt_hb<MP>() + MP; }  // forward filter,
+template <int MP>
+constexpr int czt_ch() { return czt_hf<MP>() + MP; }  // chirp w_k
+// LDS, in double2 units: the pass areas NC (MP + MP/8), which also stage the
+// grid rows and Fourier rows read and the grid rows written; in the forward
+// then the staging of a 256-order chunk of Fourier rows written (2 NC + 1
+// doubles per row: the pass areas still hold the line's Z values); then the
+// root factors.
+template <int MP, int NC>
+constexpr int czt_pa() { return NC * (MP + MP / 8); }
+template <int MP, int NC>
+constexpr int czt_st() { return 256 * (2 * NC + 1); }
+template <int MP>
+constexpr int czt_lds_bytes(bool fwd) {
+  return (int)sizeof(double2) *
+         (czt_pa<MP, czt_nc<MP>()>() + (fwd ? czt_st<MP, czt_nc<MP>()>() : 0) + czt_hb<MP>());
+}
+
+// The 8 XCDs each take a contiguous range of line blocks; the G = gridDim.x/8
+// work-groups of an XCD (dispatch deals them round-robin to the XCDs) take
+// consecutive blocks of its range, so the blocks in flight on one XCD are
+// neighbours and share the 128-byte rows of the line-contiguous layouts in
+// its L2.  gridDim.x is a multiple of 8.
+__device__ __forceinline__ void czt_range(long long nblk, long long &b, long long &bend,
+                                          long long &step) {
+  const long long x = blockIdx.x % 8, i = blockIdx.x / 8;
+  step = gridDim.x / 8;
+  b = x * nblk / 8 + i;
+  bend = (x + 1) * nblk / 8;
+}
+
+__device__ __forceinline__ double2 conjd(double2 a) { return make_double2(a.x, -a.y); }
+
+template <int MP, int NC>
+__global__ void __launch_bounds__(NC * MP / 8, 2) k_fftz_bwd(int N, int L, long long nline,
+                                                          long long nblk,
+                                                          const double *__restrict__ X,
+                                                          FourierAddr xa, double *__restrict__ v,
+                                                          const double2 *__restrict__ zt) {
+  extern __shared__ double2 lds[];
+  constexpr int T = MP / 8, NT = NC * T, W = 2 * NC, SS = W + 1;
+  // (2L+1)(2NC+1) doubles of Fourier rows fit the pass areas: 2L + 1 <=
+  // (MP + 1) / 2 since Nphi >= 2L + 2
+  constexpr int OFF = czt_pa<MP, NC>(), CR = OFF / (NC + 1);
+  const int tid = threadIdx.x, cl = tid / T, j = tid % T;
+  const int nd = blockIdx.y;
+  double2 *rts = lds + OFF;
+  for (int i = tid; i < czt_hb<MP>(); i += NT) rts[i] = zt[i];
+  const TwoLevelRoots twp{rts, rts + MP / 64};
+  const double2 *hat = zt + czt_hb<MP>(), *chirp = zt + czt_ch<MP>();
+  double *stage = reinterpret_cast<double *>(lds);
+  double2 *ln = lds + cl * (MP + MP / 8);
+  const long long nnod = nline * N;
+  const bool vec2 = (nline & 1) == 0;
+  const long long xbase = (long long)nd * xa.s_nd;
+  const int nrow = 2 * L + 1;
+  long long b, bend, step;
+  czt_range(nblk, b, bend, step);
+  for (; b < bend; b += step) {
+    const long long line0 = b * W;
+    __syncthreads();  // root factors stored; the previous block's staging read
+    // rows q = 0 .. 2L of the W lines: batches of 8 independent loads per
+    // thread (clamped into range, zeroed after), then the LDS stores
+    for (int base = 0; base < nrow * W; base += 8 * NT) {
+      double val[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int idx = base + tid + u * NT;
+        const long long line = line0 + (idx % W);
+        const bool ok = idx < nrow * W && line < nline;
+        const int row = ok ? idx / W : 0;
+        val[u] = X[q_offset(xa, row) + xbase + (ok ? line : line0) * xa.s_line];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int idx = base + tid + u * NT;
+        if (idx < nrow * W)
+          stage[(idx / W) * SS + (idx % W)] = line0 + (idx % W) < nline ? val[u] : 0.0;
+      }
+    }
+    __syncthreads();
+    // u at index k = j + T q: Z_k w_k (k <= L), Z_{N-m} w_m at k = MP - m
+    // (1 <= m <= L), zero between
+    double2 w[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int k = j + T * q;
+      double2 z = make_double2(0.0, 0.0);
+      if (k <= L) {
+        const double *p = stage + (k == 0 ? 0 : 2 * k - 1) * SS + 2 * cl;
+        if (k == 0) {
+          z = make_double2(p[0], p[1]);
+        } else {
+          const double aa = p[0], ab = p[1], ba = p[SS], bb = p[SS + 1];
+          z = make_double2(0.5 * (aa + bb), 0.5 * (ab - ba));
+        }
+        z = cmul(z, chirp[k]);
+      } else if (k >= MP - L) {
+        const int m = MP - k;
+        const double *p = stage + (2 * m - 1) * SS + 2 * cl;
+        const double aa = p[0], ab = p[1], ba = p[SS], bb = p[SS + 1];
+        z = cmul(make_double2(0.5 * (aa - bb), 0.5 * (ab + ba)), chirp[m]);
+      }
+      w[q] = z;
+    }
+    __syncthreads();  // staging read before the passes overwrite it
+    f3_passes_2l<MP, 1>(w, ln, j, twp, -1.0);
+    double2 y[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) y[q] = cmul(w[f3_pos<MP>(q)], hat[j + T * q]);
+    f3_passes_2l<MP, 1>(y, ln, j, twp, 1.0);
+    // x_i = w_i y_i for i = j + T q < N (register f3_pos(q)), staged in
+    // chunks of CR rows as [i][NC complex lines] (stride NC + 1)
+    for (int i0 = 0; i0 < N; i0 += CR) {
+      __syncthreads();  // pass areas / the previous chunk free
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int i = j + T * q;
+        if (i >= i0 && i < i0 + CR && i < N)
+          lds[(i - i0) * (NC + 1) + cl] = cmul(y[f3_pos<MP>(q)], chirp[i]);
+      }
+      __syncthreads();
+      const int nr = min(CR, N - i0);
+      for (int idx = tid; idx < nr * NC; idx += NT) {
+        const int i = idx / NC, c = idx % NC;
+        const long long la = line0 + 2 * c;
+        const double2 z = lds[i * (NC + 1) + c];
+        double *o = v + (long long)nd * nnod + (long long)(i0 + i) * nline + la;
+        if (la + 1 < nline) {
+          if (vec2)
+            *reinterpret_cast<double2 *>(o) = z;
+          else {
+            o[0] = z.x;
+            o[1] = z.y;
+          }
+        } else if (la < nline) {
+          o[0] = z.x;
+        }
+      }
+    }
+  }
+}
+
+template <int MP, int NC>
+__global__ void __launch_bounds__(NC * MP / 8, 2) k_fftz_fwd(int N, int L, long long nline,
+                                                          long long nblk,
+                                                          const double *__restrict__ v,
+                                                          double *__restrict__ X, FourierAddr xa,
+                                                          const double2 *__restrict__ zt) {
+  extern __shared__ double2 lds[];
+  constexpr int T = MP / 8, NT = NC * T, W = 2 * NC, SS = W + 1;
+  constexpr int OFF = czt_pa<MP, NC>(), CR = OFF / (NC + 1);
+  const int tid = threadIdx.x, cl = tid / T, j = tid % T;
+  const int nd = blockIdx.y;
+  double2 *rts = lds + OFF + czt_st<MP, NC>();
+  for (int i = tid; i < czt_hb<MP>(); i += NT) rts[i] = zt[i];
+  const TwoLevelRoots twp{rts, rts + MP / 64};
+  const double2 *hat = zt + czt_hf<MP>(), *chirp = zt + czt_ch<MP>();
+  double *stage = reinterpret_cast<double *>(lds + OFF);
+  double2 *ln = lds + cl * (MP + MP / 8);
+  const long long nnod = nline * N;
+  const bool vec2 = (nline & 1) == 0;
+  const long long xbase = (long long)nd * xa.s_nd;
+  const double sn = 1.0 / (double)N, tsn = 2.0 / (double)N;
+  long long b, bend, step;
+  czt_range(nblk, b, bend, step);
+  for (; b < bend; b += step) {
+    const long long line0 = b * W;
+    // u_i = x_i conj(w_i), i < N, zero up to MP; rows staged in chunks of CR
+    double2 w[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) w[q] = make_double2(0.0, 0.0);
+    for (int i0 = 0; i0 < N; i0 += CR) {
+      __syncthreads();  // root factors stored; the previous chunk / block read
+      const int nr = min(CR, N - i0);
+      for (int base = 0; base < nr * NC; base += 8 * NT) {
+        // eight independent loads per thread (lines clamped into range,
+        // zeroed after), then the LDS stores
+        double2 z[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int idx = base + tid + u * NT, c = idx % NC;
+          const int i = idx < nr * NC ? idx / NC : 0;
+          const long long la = line0 + 2 * c;
+          const double *row = v + (long long)nd * nnod + (long long)(i0 + i) * nline;
+          if (vec2)
+            z[u] = *reinterpret_cast<const double2 *>(row + (la + 1 < nline ? la : nline - 2));
+          else
+            z[u] = make_double2(row[la < nline ? la : nline - 1],
+                                row[la + 1 < nline ? la + 1 : nline - 1]);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int idx = base + tid + u * NT;
+          if (idx < nr * NC) {
+            const long long la = line0 + 2 * (idx % NC);
+            lds[(idx / NC) * (NC + 1) + (idx % NC)] =
+                make_double2(la < nline ? z[u].x : 0.0, la + 1 < nline ? z[u].y : 0.0);
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int i = j + T * q;
+        if (i >= i0 && i < i0 + nr) w[q] = cmul(lds[(i - i0) * (NC + 1) + cl], conjd(chirp[i]));
+      }
+    }
+    __syncthreads();  // staging read before the passes overwrite it
+    f3_passes_2l<MP, 1>(w, ln, j, twp, -1.0);
+    double2 y[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) y[q] = cmul(w[f3_pos<MP>(q)], hat[j + T * q]);
+    f3_passes_2l<MP, 1>(y, ln, j, twp, 1.0);
+    // Z_m = conj(w_m) y_m (m <= L) and Z_{N-m} = conj(w_m) y_{MP-m}
+    // (1 <= m <= L), into the line's pass area at indices m and MP - m (the
+    // last pass's reads are ordered before these stores)
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int k = j + T * q;
+      const double2 yk = y[f3_pos<MP>(q)];
+      if (k <= L)
+        ln[k] = cmul(yk, conjd(chirp[k]));
+      else if (k >= MP - L)
+        ln[k] = cmul(yk, conjd(chirp[MP - k]));
+    }
+    pass_lds_order<MP>();
+    for (int f0 = 0; f0 <= L; f0 += 256) {
+      const int f1 = min(f0 + 256, L + 1);
+      const int q0 = f0 == 0 ? 0 : 2 * f0 - 1, nrow = 2 * (f1 - 1) - q0 + 1;
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int m = j + T * q;
+        if (m >= f0 && m < f1) {
+          const double2 zm = ln[m], zc = ln[(MP - m) & (MP - 1)];
+          // X_a = (Z_m + conj Z_{N-m})/2, X_b = -i (Z_m - conj Z_{N-m})/2
+          const double xar = 0.5 * (zm.x + zc.x), xai = 0.5 * (zm.y - zc.y);
+          const double xbr = 0.5 * (zm.y + zc.y), xbi = -0.5 * (zm.x - zc.x);
+          if (m == 0) {
+            double *p = stage + 2 * cl;
+            p[0] = sn * xar;
+            p[1] = sn * xbr;
+          } else {
+            double *p = stage + (2 * m - 1 - q0) * SS + 2 * cl;
+            p[0] = tsn * xar;
+            p[1] = tsn * xbr;
+            p[SS] = -tsn * xai;
+            p[SS + 1] = -tsn * xbi;
+          }
+        }
+      }
+      __syncthreads();
+      for (int idx = tid; idx < nrow * W; idx += NT) {
+        const int row = idx / W, lw = idx % W;
+        const long long line = line0 + lw;
+        if (line < nline)
+          X[q_offset(xa, q0 + row) + xbase + line * xa.s_line] = stage[row * SS + lw];
+      }
+    }
+  }
+}
+
+// in-place radix-2 FFT in long double, sign -1 (exp(-2 pi i ...)); n a power of two
+static void fft_ld(std::vector<long double> &re, std::vector<long double> &im) {
+  const int n = (int)re.size();
+  for (int i = 1, jj = 0; i < n; i++) {
+    int bit = n >> 1;
+    for (; jj & bit; bit >>= 1) jj ^= bit;
+    jj ^= bit;
+    if (i < jj) {
+      std::swap(re[i], re[jj]);
+      std::swap(im[i], im[jj]);
+    }
+  }
+  const long double kPi = 3.14159265358979323846264338327950288L;
+  for (int len = 2; len <= n; len <<= 1) {
+    for (int i = 0; i < n; i += len)
+      for (int k = 0; k < len / 2; k++) {
+        const long double a = -2.0L * kPi * (long double)k / (long double)len;
+        const long double c = cosl(a), s = sinl(a);
+        const int u = i + k, t = i + k + len / 2;
+        const long double xr = re[t] * c - im[t] * s, xi = re[t] * s + im[t] * c;
+        re[t] = re[u] - xr;
+        im[t] = im[u] - xi;
+        re[u] += xr;
+        im[u] += xi;
+      }
+  }
+}
+
+int czt_init(sht_plan *p) {
+  FftPlan &fp = p->fft;
+  if (fp.zm) return SHT_OK;
+  const int N = p->nphi, L = p->L;
+  int MP = 256;
+  while (MP < N + 2 * L) MP *= 2;
+  if (MP > 2048) {
+    set_error("chirp-z form: Nphi + 2L = %d exceeds 2048", N + 2 * L);
+    return SHT_ERR_UNSUPPORTED;
+  }
+  const long double kPi = 3.14159265358979323846264338327950288L;
+  // w_k = exp(i pi k^2 / N), from k^2 mod 2N (exact)
+  auto chirp = [&](long long k, long double &c, long double &s) {
+    const long long r = (k * k) % (2LL * N);
+    const long double a = kPi * (long double)r / (long double)N;
+    c = cosl(a);
+    s = sinl(a);
+  };
+  const int hb = MP / 64 + 64, hf = hb + MP, ch = hf + MP;
+  std::vector<double2> tab((size_t)ch + N);
+  auto root = [&](int k) {  // exp(-2 pi i k / MP)
+    const long double a = -2.0L * kPi * (long double)k / (long double)MP;
+    return make_double2((double)cosl(a), (double)sinl(a));
+  };
+  for (int a = 0; a < MP / 64; a++) tab[a] = root(64 * a);
+  for (int r = 0; r < 64; r++) tab[MP / 64 + r] = root(r);
+  // backward kernel conj(w_k), k in [-L, N-1+L]; forward kernel w_k, k in
+  // [-(N-1)-L, L]; each at k mod MP; transformed and scaled by 1/MP
+  for (int dir = 0; dir < 2; dir++) {
+    std::vector<long double> re(MP, 0.0L), im(MP, 0.0L);
+    const int lo = dir == 0 ? -L : -(N - 1) - L, hi = dir == 0 ? N - 1 + L : L;
+    for (int k = lo; k <= hi; k++) {
+      long double c, s;
+      chirp(k < 0 ? -k : k, c, s);
+      const int i = ((k % MP) + MP) % MP;
+      re[i] = c;
+      im[i] = dir == 0 ? -s : s;
+    }
+    fft_ld(re, im);
+    for (int k = 0; k < MP; k++)
+      tab[(size_t)(dir == 0 ? hb : hf) + k] = make_double2((double)(re[k] / MP), (double)(im[k] / MP));
+  }
+  for (int k = 0; k < N; k++) {
+    long double c, s;
+    chirp(k, c, s);
+    tab[(size_t)ch + k] = make_double2((double)c, (double)s);
+  }
+  double2 *d = nullptr;
+  if (hipMalloc(&d, sizeof(double2) * tab.size()) != hipSuccess) {
+    set_error("chirp-z form: table of %zu bytes: out of device memory", sizeof(double2) * tab.size());
+    return SHT_ERR_OOM;
+  }
+  p->t.ztab = d;  // freed with the plan
+  p->device_bytes += (long long)(sizeof(double2) * tab.size());
+  KSHT_HIP(hipMemcpy(d, tab.data(), sizeof(double2) * tab.size(), hipMemcpyHostToDevice));
+#define KSHT_CZT_ATTR(M)                                                                     \
+  case M:                                                                                  \
+    KSHT_HIP(hipFuncSetAttribute((const void *)k_fftz_bwd<M, czt_nc<M>()>,                   \
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,               \
+                                 czt_lds_bytes<M>(false)));                                \
+    KSHT_HIP(hipFuncSetAttribute((const void *)k_fftz_fwd<M, czt_nc<M>()>,                   \
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,               \
+                                 czt_lds_bytes<M>(true)));                                 \
+    break;
+  switch (MP) {
+    KSHT_CZT_ATTR(256)
+    KSHT_CZT_ATTR(512)
+    KSHT_CZT_ATTR(1024)
+    KSHT_CZT_ATTR(2048)
+    default: break;
+  }
+#undef KSHT_CZT_ATTR
+  fp.zm = MP;
+  return SHT_OK;
+}
+
+// work-groups per launch: 8 x G, about two rounds of what the CUs hold
+static unsigned czt_grid_x(long long nblk, int ncomp, int lds_bytes) {
+  const int per_cu = lds_bytes > 80 * 1024 ? 1 : lds_bytes > 53 * 1024 ? 2 : 3;
+  long long g = (2LL * 256 * per_cu) / (8LL * (ncomp > 0 ? ncomp : 1));
+  if (g < 1) g = 1;
+  const long long need = (nblk + 7) / 8;
+  if (g > need) g = need;
+  return (unsigned)(8 * g);
+}
+
+static int launch_czt(sht_plan *p, int ncomp, bool bwd, const double *in, double *out,
+                      FourierAddr xa) {
+  const long long nline = (long long)p->nr * p->nth;
+#define KSHT_CZT_CASE(M)                                                                     \
+  case M: {                                                                                \
+    constexpr int nc_ = czt_nc<M>();                                                       \
+    const long long nblk = (nline + 2 * nc_ - 1) / (2 * nc_);                              \
+    const int lds_ = czt_lds_bytes<M>(!bwd);                                               \
+    const dim3 g((unsigned)czt_grid_x(nblk, ncomp, lds_), (unsigned)ncomp);                  \
+    if (bwd)                                                                               \
+      hipLaunchKernelGGL((k_fftz_bwd<M, nc_>), g, dim3(M / 8 * nc_), lds_, p->stream,        \
+                         p->nphi, p->L, nline, nblk, in, xa, out, p->t.ztab);              \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_fftz_fwd<M, nc_>), g, dim3(M / 8 * nc_), lds_, p->stream,        \
+                         p->nphi, p->L, nline, nblk, in, out, xa, p->t.ztab);              \
+  } break;
+  switch (p->fft.zm) {
+    KSHT_CZT_CASE(256)
+    KSHT_CZT_CASE(512)
+    KSHT_CZT_CASE(1024)
+    KSHT_CZT_CASE(2048)
+    default:
+      set_error("chirp-z form: not set up for this plan");
+      return SHT_ERR_ARG;
+  }
+#undef KSHT_CZT_CASE
+  KSHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+
 int fft_plan_init(sht_plan *p) {
   FftPlan &fp = p->fft;
   const int n = p->nphi;
@@ -1128,6 +1614,7 @@ static bool dft_applies(const sht_plan *p, const FourierAddr &xa) {
 
 int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, double *v) {
   if (dft_applies(p, xa)) return launch_dft_bwd(p, ncomp, X, xa, v);
+  if (p->var_fft_form == 3 && p->fft.zm) return launch_czt(p, ncomp, true, X, v, xa);
   const long long nline = (long long)p->nr * p->nth;
   const int B = p->fft.lines_per_wg;
   const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);
@@ -1167,6 +1654,7 @@ int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, doub
 
 int launch_fft_fwd(sht_plan *p, int ncomp, const double *v, double *X, FourierAddr xa) {
   if (dft_applies(p, xa)) return launch_dft_fwd(p, ncomp, v, X, xa);
+  if (p->var_fft_form == 3 && p->fft.zm) return launch_czt(p, ncomp, false, v, X, xa);
   const long long nline = (long long)p->nr * p->nth;
   const int B = p->fft.lines_per_wg;
   const dim3 grid((unsigned)((nline + B - 1) / B), (unsigned)ncomp);

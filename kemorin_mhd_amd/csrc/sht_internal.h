@@ -52,6 +52,9 @@ struct FftPlan {
   // split form n = r0 * np (np a power of two above L)
   int fast, np, r0, lds3, lds3b;  // LDS bytes: forward, backward (whole-block staging)
   unsigned r0m;                   // ceil(2^32 / r0): division by r0 as a multiply-high
+  // chirp-z form (set up on first selection): convolution length, 0 when
+  // not set up
+  int zm = 0;
 };
 
 // Work-group order of the GEMMs.  The hardware deals work-groups round-robin
@@ -109,6 +112,7 @@ struct Tables {
   double *cos_t = nullptr, *sin_t = nullptr;    // device, Ntheta
   double2 *tw = nullptr;                        // FFT twiddles
   double2 *twn = nullptr;                       // exp(-2 pi i k / Nphi), k < Nphi
+  double2 *ztab = nullptr;                      // chirp-z form: roots, filters, chirp
   long long *slot_off_X = nullptr, *slot_off_rtm = nullptr;  // 2L+1 each
   double *pole = nullptr;  // P_l^0, dP_l^1/dtheta at theta = 0 and pi, 4 (L+1)
 };
@@ -208,6 +212,7 @@ int launch_fft_fwd(sht_plan *p, int ncomp, const double *v, double *X,
                    FourierAddr xa);
 // zonal transform as a DFT product (sht_dft.hip)
 int dft_init(sht_plan *p);
+int czt_init(sht_plan *p);
 void dft_free(sht_plan *p);
 int launch_dft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, double *v);
 int launch_dft_fwd(sht_plan *p, int ncomp, const double *v, double *X, FourierAddr xa);

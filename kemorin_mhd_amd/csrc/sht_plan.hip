@@ -357,7 +357,7 @@ static void free_plan(sht_plan *p) {
   if (!p) return;
   void *ptrs[] = {p->t.offb, p->t.offf, p->t.ipad, p->t.jst, p->t.Pb, p->t.dPb,
                   p->t.Pf, p->t.dPf, p->t.radius, p->t.a_r, p->t.weight, p->t.asin_t,
-                  p->t.cos_t, p->t.sin_t, p->t.tw, p->t.twn, p->t.slot_off_X, p->t.slot_off_rtm,
+                  p->t.cos_t, p->t.sin_t, p->t.tw, p->t.twn, p->t.ztab, p->t.slot_off_X, p->t.slot_off_rtm,
                   p->t.pole, p->X, p->offb_r, p->offf_r, p->Pt, p->dPt};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
@@ -522,7 +522,7 @@ extern "C" long long sht_plan_device_bytes(const sht_plan *p) { return p ? p->de
 
 // Kernel variants: name -> the plan field and its allowed values.
 static int *variant_slot(sht_plan *p, const char *name, const int **allowed, int *nallowed) {
-  static const int mfma[] = {16, 32}, form[] = {0, 1, 2};
+  static const int mfma[] = {16, 32}, form[] = {0, 1, 2, 3};
   if (!strcmp(name, "split_fwd_mfma")) {
     *allowed = mfma;
     *nallowed = 2;
@@ -530,7 +530,7 @@ static int *variant_slot(sht_plan *p, const char *name, const int **allowed, int
   }
   if (!strcmp(name, "fft_form")) {
     *allowed = form;
-    *nallowed = 3;
+    *nallowed = 4;
     return &p->var_fft_form;
   }
   return nullptr;
@@ -550,9 +550,9 @@ extern "C" int sht_plan_set_variant(sht_plan *p, const char *name, int value) {
   }
   for (int i = 0; i < n; i++)
     if (allowed[i] == value) {
-      // the DFT form sets up its bases and GEMM handle on first selection
-      if (slot == &p->var_fft_form && value == 2) {
-        const int rc = dft_init(p);
+      // the DFT and chirp-z forms set up their tables on first selection
+      if (slot == &p->var_fft_form && (value == 2 || value == 3)) {
+        const int rc = value == 2 ? dft_init(p) : czt_init(p);
         if (rc) return rc;
       }
       *slot = value;

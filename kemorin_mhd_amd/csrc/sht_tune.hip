@@ -7,9 +7,11 @@
 // fastest for the run.  Here the variants are the kernel forms of one plan:
 //   split_fwd_mfma  16 or 32: INT8 MFMA form of the forward split products
 //                   (split-precision plans);
-//   fft_form        1, 0 or 2: split-form FFT (n = r0 2^k, where the length
-//                   has it), mixed-radix Stockham FFT, or the pruned real
-//                   DFT as an FP64 GEMM (sht_dft.hip).
+//   fft_form        1, 0, 2 or 3: split-form FFT (n = r0 2^k, where the
+//                   length has it), mixed-radix Stockham FFT, the pruned real
+//                   DFT as an FP64 GEMM (sht_dft.hip), or the whole line as
+//                   one chirp-z (Bluestein) convolution of power-of-two
+//                   length (sht_fft.hip).
 // Every combination runs one backward and one forward transform untimed,
 // then the best of three timed round trips (HIP events on the plan's
 // stream) decides.  Results are cached in a JSON file, one entry per line:
@@ -141,6 +143,7 @@ extern "C" int sht_plan_autotune(sht_plan *p, int nvb, int nsb, int nvf, int nsf
   if (p->precision == SHT_PREC_SPLIT_INT8) mfmas.push_back(32);
   if (p->fft.fast) forms.push_back(0);
   if (dft_init(p) == SHT_OK) forms.push_back(2);  // the DFT product, where its set-up fits
+  if (czt_init(p) == SHT_OK) forms.push_back(3);  // the chirp-z form, Nphi + 2L <= 2048
   const int keep_mfma = p->var_fwd_mfma, keep_form = p->var_fft_form;
   const long long nnod = (long long)p->nr * p->nth * p->nphi;
   const long long nsp = (long long)p->jmax * p->nr;
