@@ -336,6 +336,46 @@ int sht_batch_launch(sht_batch *batch, void *hip_stream);
 int sht_batch_synchronize(sht_batch *batch);
 int sht_batch_destroy(sht_batch *batch);
 
+/* ---- Plans from the reference's grid / mode files ---------------------
+ * The reference reads a rank's spherical grids and modes from the four
+ * ASCII files gen_sph_grids writes (rtp, rtm, rlm, rj): read_geom_rtp_data,
+ * read_geom_rtm_data, read_spectr_modes_rlm_data, read_spectr_modes_rj_data
+ * (MHD/Fortran_src/CORE_src/IO/sph_modes_grids_data_IO.f90:53-183), copied
+ * into the grid types by copy_sph_node_4_*_from_IO
+ * (MHD/Fortran_src/CORE_src/SPH_SPECTR_src/copy_sph_node_4_IO.f90:119-281).
+ * sht_sph_files_read parses such a set (domain, sizes, truncation,
+ * subdomain indices, 1-D index tables with the radii, node lists, import
+ * tables; groups are not read); SHT_ERR_ARG with file:line on a malformed
+ * file. */
+typedef struct sht_sph_files sht_sph_files;
+int sht_sph_files_read(const char *rtp_path, const char *rtm_path, const char *rlm_path,
+                       const char *rj_path, sht_sph_files **files);
+void sht_sph_files_destroy(sht_sph_files *files);
+/* SHT_OK when the set is one subdomain per direction of the single-rank
+ * layout this engine computes in (m_folding = 1): every index table and
+ * node list as gen_sph_grids builds them for one rank, rlm and rtm import
+ * tables in node order, and rj / rtp import tables that pair each rlm / rtm
+ * node with the rj / rtp node of the same radius and mode / colatitude and
+ * FFT slot.  Otherwise SHT_ERR_UNSUPPORTED, the message naming the file and
+ * the first entry that differs. */
+int sht_sph_files_check(const sht_sph_files *files);
+/* dims = {L, Nr, Ntheta, Nphi, nnod_rj}; radius (NULL = skip) gets Nr radii. */
+int sht_sph_files_shape(const sht_sph_files *files, int dims[5], double *radius);
+/* A table of the set by name: "<space>_import" (1-based node numbers in
+ * buffer order; the reference's item_sr), "<space>_import_rev" (node ->
+ * 1-based buffer record, ntot + 1 for a node not in the table; irev_sr),
+ * "<space>_nodes" (global indices per node), "<space>_table2" /
+ * "<space>_table3" (the 1-D tables); space = rtp, rtm, rlm or rj.  Copies
+ * into out (NULL: only the length) and returns the number of ints, or minus
+ * an error code.  The rj tables drive sht_rj_fields_to_send /
+ * sht_rj_fields_from_recv. */
+long long sht_sph_files_table(const sht_sph_files *files, const char *name, int *out,
+                              long long n);
+/* sht_plan_create for the shape and radii of a set that passes
+ * sht_sph_files_check (its error otherwise). */
+int sht_plan_create_from_files(const sht_sph_files *files, int max_ncomp, int precision,
+                               int n_slices, unsigned flags, sht_plan **plan);
+
 /* Diagnostics: checks the MFMA lane layouts this library assumes on the
  * current device (FP64 16x16x4 and INT8 16x16x64).  0 = as assumed. */
 int sht_selftest(void);

@@ -15,7 +15,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkemorin_sht.so")
 SOURCES = ["sht_plan.hip", "sht_legendre.hip", "sht_fft.hip", "sht_reorder.hip",
            "sht_split.hip", "sht_capi.hip", "sht_batch.hip", "sht_pole.hip",
-           "sht_nonlinear.hip", "sht_copy.hip", "sht_coriolis.hip", "sht_tune.hip", "sht_dft.hip"]
+           "sht_nonlinear.hip", "sht_copy.hip", "sht_coriolis.hip", "sht_tune.hip", "sht_dft.hip",
+           "sht_files.hip"]
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-fno-gpu-rdc",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -119,6 +119,15 @@ def lib() -> ctypes.CDLL:
         L.sht_batch_launch.argtypes = [_vp, _vp]
         L.sht_batch_synchronize.argtypes = [_vp]
         L.sht_batch_destroy.argtypes = [_vp]
+        L.sht_sph_files_read.argtypes = [ctypes.c_char_p] * 4 + [ctypes.POINTER(_vp)]
+        L.sht_sph_files_destroy.argtypes = [_vp]
+        L.sht_sph_files_destroy.restype = None
+        L.sht_sph_files_check.argtypes = [_vp]
+        L.sht_sph_files_shape.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), _dp]
+        L.sht_sph_files_table.argtypes = [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), _ll]
+        L.sht_sph_files_table.restype = _ll
+        L.sht_plan_create_from_files.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_uint, ctypes.POINTER(_vp)]
         L.sht_get_last_error.argtypes = [ctypes.c_char_p, ctypes.c_int]
         L.sht_version.restype = ctypes.c_char_p
         _LIB = L
@@ -222,6 +231,18 @@ class SphTransPlan:
         self.max_ncomp = max_ncomp
         self.precision = precision
         self._user_stream = False
+
+    @classmethod
+    def _adopt(cls, h, cfg, radius, max_ncomp, precision):
+        """Wrap a plan handle created elsewhere (sht_plan_create_from_files)."""
+        self = cls.__new__(cls)
+        self.cfg = cfg
+        self._radius = np.ascontiguousarray(radius, dtype=np.float64)
+        self._h = h
+        self.max_ncomp = max_ncomp
+        self.precision = precision
+        self._user_stream = False
+        return self
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
@@ -504,6 +525,68 @@ class CoriolisRlm:
             self._h, ncomp_trans, i_rot_coriolis, _ptr(d_cor),
             item.data_ptr() if item is not None else None, nnod_send, _ptr(WS)),
             "sht_copy_coriolis_terms_rlm")
+
+
+class SphFiles:
+    """A rank's spherical grid / mode files as gen_sph_grids writes them (rtp,
+    rtm, rlm, rj), read by sht_sph_files_read: mirrors the reference's
+    read_geom_rtp_data / read_geom_rtm_data / read_spectr_modes_rlm_data /
+    read_spectr_modes_rj_data (CORE_src/IO/sph_modes_grids_data_IO.f90:53-183)."""
+
+    SPACES = ("rtp", "rtm", "rlm", "rj")
+
+    def __init__(self, rtp: str, rtm: str, rlm: str, rj: str):
+        h = _vp()
+        _check(lib().sht_sph_files_read(rtp.encode(), rtm.encode(), rlm.encode(), rj.encode(),
+                                        ctypes.byref(h)), "sht_sph_files_read")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().sht_sph_files_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self):
+        """Raise ShtError unless the set is the single-rank layout."""
+        _check(lib().sht_sph_files_check(self._h), "sht_sph_files_check")
+
+    def shape(self):
+        """(ShellConfig, radii, nnod_rj) of the set."""
+        dims = (ctypes.c_int * 5)()
+        _check(lib().sht_sph_files_shape(self._h, dims, None), "sht_sph_files_shape")
+        r = np.empty(dims[1])
+        _check(lib().sht_sph_files_shape(self._h, dims, r.ctypes.data_as(_dp)),
+               "sht_sph_files_shape")
+        return ShellConfig(dims[0], dims[1], dims[2], dims[3]), r, dims[4]
+
+    def table(self, name: str) -> np.ndarray:
+        """A table of the set (sht_sph_files_table), as int32."""
+        n = lib().sht_sph_files_table(self._h, name.encode(), None, 0)
+        if n < 0:
+            raise ShtError(int(-n), "sht_sph_files_table")
+        out = np.empty(n, dtype=np.int32)
+        if n:
+            m = lib().sht_sph_files_table(self._h, name.encode(),
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n)
+            if m < 0:
+                raise ShtError(int(-m), "sht_sph_files_table")
+        return out
+
+    def plan(self, max_ncomp: int = 9, precision: int = SHT_PREC_FP64, n_slices: int = 0,
+             p_on_the_fly: bool = False) -> "SphTransPlan":
+        """The plan of this set's shape and radii (sht_plan_create_from_files)."""
+        cfg, r, _ = self.shape()
+        h = _vp()
+        _check(lib().sht_plan_create_from_files(self._h, max_ncomp, precision, n_slices,
+                                                SHT_FLAG_P_ON_THE_FLY if p_on_the_fly else 0,
+                                                ctypes.byref(h)), "sht_plan_create_from_files")
+        return SphTransPlan._adopt(h, cfg, r, max_ncomp, precision)
 
 
 class ShellBatch:

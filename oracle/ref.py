@@ -47,6 +47,7 @@ def lib():
         L.ref_from_recv_rev.argtypes = [ctypes.c_int] * 4 + [_ip] + [ctypes.c_int] * 3 + [_dp, _dp]
         L.ref_coriolis_tables.argtypes = [_ip, _ip] + [_dp] * 8
         L.ref_sum_coriolis.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double, ctypes.c_int, _dp, _dp]
+        L.ref_write_sph_files.argtypes = [ctypes.c_int] * 4 + [_dp, _ip, ctypes.c_char_p]
         _LIB = L
     return _LIB
 
@@ -191,3 +192,20 @@ class RefPlan:
         assert out.size == (3 * nv + ns) * (2 * self.nr + 1)
         lib().ref_center(nv, ns, _p(WR), _p(out))
         return out
+
+
+def write_sph_files(L, nr, nth, nphi, radius, paths, item_rj=None):
+    """A one-rank set of grid / mode files (rtp, rtm, rlm, rj at the four
+    paths) written by the reference's own writers with its own ordering
+    routines (oracle/ref_sph_files.f90); item_rj: the rj import items
+    (1-based rj node per rlm node; None = node order)."""
+    jmax = (L + 1) ** 2
+    if item_rj is None:
+        item_rj = np.arange(1, jmax * nr + 1)
+    item = np.ascontiguousarray(item_rj, dtype=np.int32)
+    assert item.size == jmax * nr
+    buf = b"".join(p.encode().ljust(512, b"\0")[:511] + b"\0" for p in paths)
+    assert len(buf) == 4 * 512
+    r = np.ascontiguousarray(radius, dtype=np.float64)
+    lib().ref_write_sph_files(L, nr, nth, nphi, _p(r),
+                              item.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), buf)
