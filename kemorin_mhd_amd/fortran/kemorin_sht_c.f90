@@ -330,6 +330,54 @@
           integer(c_int), value :: n
         end function sht_get_last_error
 !
+!       Plans from the grid / mode files (sht_files.hip); paths are
+!       NUL-terminated (trim(path) // c_null_char)
+        integer(c_int) function sht_sph_files_read(rtp, rtm, rlm, rj,  &
+     &          files) bind(C, name = 'sht_sph_files_read')
+          import :: c_int, c_ptr, c_char
+          character(kind = c_char), intent(in) :: rtp(*), rtm(*)
+          character(kind = c_char), intent(in) :: rlm(*), rj(*)
+          type(c_ptr), intent(out) :: files
+        end function sht_sph_files_read
+!
+        subroutine sht_sph_files_destroy(files)                        &
+     &                          bind(C, name = 'sht_sph_files_destroy')
+          import :: c_ptr
+          type(c_ptr), value :: files
+        end subroutine sht_sph_files_destroy
+!
+        integer(c_int) function sht_sph_files_check(files)             &
+     &                          bind(C, name = 'sht_sph_files_check')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: files
+        end function sht_sph_files_check
+!
+        integer(c_int) function sht_sph_files_shape(files, dims,       &
+     &          radius) bind(C, name = 'sht_sph_files_shape')
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: files
+          integer(c_int), intent(out) :: dims(5)
+          real(c_double), intent(out) :: radius(*)
+        end function sht_sph_files_shape
+!
+        integer(c_long_long) function sht_sph_files_table(files, name, &
+     &          out, n) bind(C, name = 'sht_sph_files_table')
+          import :: c_ptr, c_char, c_int, c_long_long
+          type(c_ptr), value :: files
+          character(kind = c_char), intent(in) :: name(*)
+          integer(c_int), intent(out) :: out(*)
+          integer(c_long_long), value :: n
+        end function sht_sph_files_table
+!
+        integer(c_int) function sht_plan_create_from_files(files,      &
+     &          max_ncomp, precision, n_slices, flags, plan)            &
+     &                    bind(C, name = 'sht_plan_create_from_files')
+          import :: c_int, c_ptr
+          type(c_ptr), value :: files
+          integer(c_int), value :: max_ncomp, precision, n_slices, flags
+          type(c_ptr), intent(out) :: plan
+        end function sht_plan_create_from_files
+!
         type(c_ptr) function sht_version()                             &
      &                          bind(C, name = 'sht_version')
           import :: c_ptr

@@ -13,6 +13,7 @@ import pytest
 
 import kemorin_mhd_amd as K
 from kemorin_mhd_amd import sht
+from tests.golden.make_sph_files import paths as sph_fixture_paths
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "kemorin_sht.h")
@@ -151,9 +152,11 @@ _FORTRAN_DRIVER = """
       implicit none
       type(sht_desc) :: d
       type(sht_nonlinear_desc) :: nl
-      type(c_ptr) :: plan
+      type(c_ptr) :: plan, files
       real(c_double), target :: r(4)
-      integer(c_int) :: rc, icase
+      real(c_double) :: rr(16)
+      integer(c_int) :: rc, rc2, icase, dims(5), items(200)
+      integer(c_long_long) :: nitem
       r = (/0.5d0, 0.8d0, 1.1d0, 1.5d0/)
       do icase = 0, 10
         d%l_truncation = 3
@@ -197,6 +200,18 @@ _FORTRAN_DRIVER = """
       rc = sht_nonlinear_rtp(c_null_ptr, nl, 3_c_int, c_null_ptr,       &
      &                       3_c_int, c_null_ptr)
       write(*, '(a,i0)') 'nl=', rc
+      rc = sht_sph_files_read('@RTP@' // c_null_char,                   &
+     &    '@RTM@' // c_null_char, '@RLM@' // c_null_char,               &
+     &    '@RJ@' // c_null_char, files)
+      write(*, '(a,i0)') 'files=', rc
+      rc = sht_sph_files_check(files)
+      rc2 = sht_sph_files_shape(files, dims, rr)
+      nitem = sht_sph_files_table(files, 'rj_import' // c_null_char,   &
+     &                            items, 200_c_long_long)
+      write(*, '(a,i0,a,i0,a,5(1x,i0),a,i0,a,i0,a,i0)') 'check=', rc,      &
+     &    ' shape=', rc2, ' dims', dims, ' items=', nitem,              &
+     &    ' first=', items(1), ' last=', items(nitem)
+      call sht_sph_files_destroy(files)
       contains
       function last_error() result(msg)
       character(len = 256) :: msg
@@ -232,7 +247,10 @@ def test_fortran_binding_lays_out_every_descriptor_field(tmp_path):
     validation (and then finds no device here: SHT_ERR_HIP)."""
     mod = os.path.join(ROOT, "kemorin_mhd_amd", "fortran", "kemorin_sht_c.f90")
     prog = tmp_path / "drv.f90"
-    prog.write_text(_FORTRAN_DRIVER)
+    drv = _FORTRAN_DRIVER
+    for tag, path in zip(("@RTP@", "@RTM@", "@RLM@", "@RJ@"), sph_fixture_paths()):
+        drv = drv.replace(tag, path)
+    prog.write_text(drv)
     exe = tmp_path / "drv"
     libdir = os.path.dirname(sht.LIB_PATH)
     r = subprocess.run([AMDFLANG, "-J", str(tmp_path), mod, str(prog), "-o", str(exe),
@@ -248,6 +266,9 @@ def test_fortran_binding_lays_out_every_descriptor_field(tmp_path):
         assert " rc=1 " in lines[case], (field, lines[case])
         assert text in lines[case], (field, lines[case])
     assert "nl=1" in out.stdout, out.stdout + out.stderr
+    # the grid / mode files through the binding (no device needed)
+    assert "files=0" in out.stdout, out.stdout + out.stderr
+    assert "check=0 shape=0 dims 5 3 9 18 108 items=108 first=1 last=108" in out.stdout, out.stdout
 
 
 # --- the split GEMMs' hand-counted waits (checker of the generated code) ---
