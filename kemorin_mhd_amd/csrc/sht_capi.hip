@@ -211,7 +211,7 @@ static int host_call(sht_plan *p, int nv, int ns, const double *in, double *out,
   if (!rc) rc = bwd ? sht_backward(p, nv, ns, din, dout) : sht_forward(p, nv, ns, din, dout);
   if (!rc && hipMemcpyAsync(out, dout, bout, hipMemcpyDeviceToHost, p->stream) != hipSuccess)
     rc = SHT_ERR_HIP;
-  if (hipStreamSynchronize(p->stream) != hipSuccess && !rc) rc = SHT_ERR_HIP;
+  if (hipStreamSynchronize(api_stream(p->stream)) != hipSuccess && !rc) rc = SHT_ERR_HIP;
   if (rc == SHT_ERR_HIP) set_error("host transform: %s", hipGetErrorString(hipGetLastError()));
   (void)hipFree(din);
   (void)hipFree(dout);
@@ -298,7 +298,7 @@ extern "C" int sht_plan_timer_read(sht_plan *p, double *ms, long long *calls, in
     set_error("sht_plan_timer_read: bad arguments");
     return SHT_ERR_ARG;
   }
-  KSHT_HIP(hipStreamSynchronize(p->stream));
+  KSHT_HIP(hipStreamSynchronize(api_stream(p->stream)));
   int rc = SHT_OK;
   for (int s = 0; s < kNumStages; s++) {
     double tot = 0.0;

@@ -36,9 +36,7 @@ struct DftState {
 };
 
 // the plan's stream for rocBLAS (the legacy default stream as handle 0)
-static hipStream_t blas_stream(const sht_plan *p) {
-  return p->stream == hipStreamLegacy ? (hipStream_t)0 : p->stream;
-}
+static hipStream_t blas_stream(const sht_plan *p) { return api_stream(p->stream); }
 
 static int blas_ok(rocblas_status s, const char *what) {
   if (s == rocblas_status_success) return SHT_OK;

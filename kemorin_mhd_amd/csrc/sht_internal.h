@@ -125,6 +125,10 @@ struct sht_plan {
   int device;
   hipStream_t stream;
   hipStream_t own_stream;
+  // side stream (non-blocking) and fork / join events: a call's scalar-group
+  // Legendre launch runs there beside its vector group's (ForkJoin)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<double> radius, gpoint, weight, colat, asin_t;
   std::vector<int> lstack;
   std::vector<long long> h_offb, h_offf;
@@ -160,17 +164,51 @@ struct sht_plan {
 
 namespace ksht {
 // Brackets one stage launch with events when the plan's timers are on.
+// The plan's stream as HIP's event and stream-wait calls take it: the legacy
+// default stream is bound as the sentinel hipStreamLegacy (kernel launches
+// accept it), which these calls need as the null stream instead.
+inline hipStream_t api_stream(hipStream_t s) { return s == hipStreamLegacy ? (hipStream_t)0 : s; }
+
+// Fork / join of the plan's side stream around launches that may overlap:
+// the side stream waits for everything queued on the plan's stream so far,
+// and the plan's stream waits for the side stream's launches at the end of
+// the scope.  Off (all on the plan's stream) when on is false, the plan has
+// no side stream, or the plan's stream is being captured into a graph (a
+// shell batch's graph runs its shells on parallel branches instead; forking
+// the plan's side stream inside that capture crashed sht_batch_create, a
+// cause not established, so the capture keeps the single-stream order).
+struct ForkJoin {
+  sht_plan *p;
+  bool on;
+  ForkJoin(sht_plan *p_, bool want) : p(p_), on(want && p_->side) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (on && (hipStreamIsCapturing(api_stream(p->stream), &cs) != hipSuccess ||
+               cs != hipStreamCaptureStatusNone))
+      on = false;
+    if (on && (hipEventRecord(p->ev_fork, api_stream(p->stream)) != hipSuccess ||
+               hipStreamWaitEvent(p->side, p->ev_fork, 0) != hipSuccess))
+      on = false;  // ordering not set up: stay on the plan's stream
+  }
+  hipStream_t stream(bool beside) const { return on && beside ? p->side : p->stream; }
+  ~ForkJoin() {
+    if (on) {
+      (void)hipEventRecord(p->ev_join, p->side);
+      (void)hipStreamWaitEvent(api_stream(p->stream), p->ev_join, 0);
+    }
+  }
+};
+
 struct StageTimer {
   sht_plan *p;
   int id;
   hipEvent_t a = nullptr, b = nullptr;
   StageTimer(sht_plan *p_, int id_) : p(p_), id(id_) {
     if (p->timing && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
-      (void)hipEventRecord(a, p->stream);
+      (void)hipEventRecord(a, api_stream(p->stream));
   }
   ~StageTimer() {
     if (b) {
-      (void)hipEventRecord(b, p->stream);
+      (void)hipEventRecord(b, api_stream(p->stream));
       p->tev[id].push_back({a, b});
     } else if (a) {
       (void)hipEventDestroy(a);

@@ -463,9 +463,13 @@ int launch_leg_bwd(sht_plan *p, int nv, int ns, const double *sp, SpecAddr sa, d
     PTile pt;
     int rc = ptile_prepare(p, ti, false, &pt);
     if (rc) return rc;
+    // the scalar groups run beside the vector groups (their work-groups
+    // fill the tail of the vector launch)
+    ForkJoin fj(p, nv > 0 && nsg > 0);
     for (int vec = 1; vec >= 0; vec--) {
       const int ncg = vec ? nv : nsg;
       if (ncg == 0) continue;
+      const hipStream_t st = fj.stream(!vec);
       LegDims d = make_dims(p, nv, ns, p->nth_hp / 64);
       d.ncg = ncg;
       d.M0 = pt.M0;
@@ -474,11 +478,11 @@ int launch_leg_bwd(sht_plan *p, int nv, int ns, const double *sp, SpecAddr sa, d
       const int cg0 = vec ? 0 : nv;
       if (vec) {
         StageTimer tk(p, 5);
-        hipLaunchKernelGGL(k_leg_bwd<true>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d,
+        hipLaunchKernelGGL(k_leg_bwd<true>, dim3((unsigned)nblocks), dim3(256), 0, st, d,
                            cg0, sp, sa, out, ga, p->t.jst, pt.offb, p->t.ipad, pt.Pb, pt.dPb,
                            p->t.a_r, p->t.asin_t);
       } else
-        hipLaunchKernelGGL(k_leg_bwd<false>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d,
+        hipLaunchKernelGGL(k_leg_bwd<false>, dim3((unsigned)nblocks), dim3(256), 0, st, d,
                            cg0, sp, sa, out, ga, p->t.jst, pt.offb, p->t.ipad, pt.Pb, pt.dPb,
                            p->t.a_r, p->t.asin_t);
       KSHT_HIP(hipGetLastError());
@@ -495,9 +499,11 @@ int launch_leg_fwd(sht_plan *p, int nv, int ns, const double *in, GridAddr ga, d
     PTile pt;
     int rc = ptile_prepare(p, ti, true, &pt);
     if (rc) return rc;
+    ForkJoin fj(p, nv > 0 && nsg > 0);  // scalar groups beside the vector groups
     for (int vec = 1; vec >= 0; vec--) {
       const int ncg = vec ? nv : nsg;
       if (ncg == 0) continue;
+      const hipStream_t st = fj.stream(!vec);
       LegDims d = make_dims(p, nv, ns, (ne_max + 63) / 64);
       d.ncg = ncg;
       d.M0 = pt.M0;
@@ -506,11 +512,11 @@ int launch_leg_fwd(sht_plan *p, int nv, int ns, const double *in, GridAddr ga, d
       const int cg0 = vec ? 0 : nv;
       if (vec) {
         StageTimer tk(p, 6);
-        hipLaunchKernelGGL(k_leg_fwd<true>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d,
+        hipLaunchKernelGGL(k_leg_fwd<true>, dim3((unsigned)nblocks), dim3(256), 0, st, d,
                            cg0, in, ga, sp, sa, p->t.jst, pt.offf, p->t.ipad, pt.Pf, pt.dPf,
                            p->t.radius, p->t.weight, p->t.asin_t);
       } else
-        hipLaunchKernelGGL(k_leg_fwd<false>, dim3((unsigned)nblocks), dim3(256), 0, p->stream, d,
+        hipLaunchKernelGGL(k_leg_fwd<false>, dim3((unsigned)nblocks), dim3(256), 0, st, d,
                            cg0, in, ga, sp, sa, p->t.jst, pt.offf, p->t.ipad, pt.Pf, pt.dPf,
                            p->t.radius, p->t.weight, p->t.asin_t);
       KSHT_HIP(hipGetLastError());

@@ -369,6 +369,9 @@ static void free_plan(sht_plan *p) {
       (void)hipEventDestroy(e.second);
     }
   if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
+  if (p->side) (void)hipStreamDestroy(p->side);
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  if (p->ev_join) (void)hipEventDestroy(p->ev_join);
   delete p;
 }
 
@@ -440,6 +443,14 @@ extern "C" int sht_plan_create(const sht_desc *d, sht_plan **out) {
     return SHT_ERR_HIP;
   }
   p->stream = p->own_stream;
+  // side stream for overlapping launches; without it calls run in order
+  if (hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) != hipSuccess) {
+    if (p->side) (void)hipStreamDestroy(p->side);
+    p->side = nullptr;
+    (void)hipGetLastError();
+  }
   // the fixed workspace of sht_backward_noalloc on this device; a failure
   // here is reported by that entry point, which tries again
   (void)scratch_attach(p->device);
@@ -491,7 +502,7 @@ extern "C" int sht_plan_create(const sht_desc *d, sht_plan **out) {
   }
   p->device_bytes += (long long)sizeof(double) * p->x_elems;
   if (p->precision == SHT_PREC_SPLIT_INT8) CHECK(split_init(p));
-  if (hipStreamSynchronize(p->stream) != hipSuccess) {
+  if (hipStreamSynchronize(api_stream(p->stream)) != hipSuccess) {
     set_error("sht_plan_create: table generation failed: %s",
               hipGetErrorString(hipGetLastError()));
     free_plan(p);
@@ -504,7 +515,7 @@ extern "C" int sht_plan_create(const sht_desc *d, sht_plan **out) {
 
 extern "C" int sht_plan_destroy(sht_plan *p) {
   if (!p) return SHT_OK;
-  (void)hipStreamSynchronize(p->stream);
+  (void)hipStreamSynchronize(api_stream(p->stream));
   free_plan(p);
   return SHT_OK;
 }

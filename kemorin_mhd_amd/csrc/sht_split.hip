@@ -1736,7 +1736,11 @@ static int bwd_split_S(sht_plan *p, const Workspace &ws, int nv, int ns, const d
     int M0 = 0, nM = 0;
     int rc = split_tile<S>(p, st, ti, false, &tb, &M0, &nM);
     if (rc) return rc;
+    // the scalar groups run beside the vector group (their work-groups fill
+    // the tail of the vector launch)
+    ForkJoin fj(p, nv > 0 && ng > 1);
     for (int i = 0; i < ng; i++) {
+      const hipStream_t sm = fj.stream(!g[i].vec);
       SplitDims d = split_dims(p, ncomp, p->nth_hp / 64);
       d.ncg = g[i].ncg;
       d.M0 = M0;
@@ -1744,7 +1748,7 @@ static int bwd_split_S(sht_plan *p, const Workspace &ws, int nv, int ns, const d
       const long long nblocks = (long long)nM * d.n_rt * d.ncg * d.nblk;
       const dim3 grid((unsigned)nblocks), blk(kGemmThreads);
 #define KSHT_BWD(NB, VEC)                                                                      \
-  hipLaunchKernelGGL((k_leg_bwd_split<S, NB, VEC>), grid, blk, bwd_lds<S>(NB), p->stream, d, \
+  hipLaunchKernelGGL((k_leg_bwd_split<S, NB, VEC>), grid, blk, bwd_lds<S>(NB), sm, d,        \
                      g[i].nd_first, tb, st->cb, ws.dig, ws.dexp, out, ga, p->t.a_r,          \
                      p->t.asin_t)
       if (g[i].vec) {
@@ -1780,7 +1784,9 @@ static int fwd_split_S(sht_plan *p, const Workspace &ws, int nv, int ns, const d
     int M0 = 0, nM = 0;
     int rc = split_tile<S>(p, st, ti, true, &tb, &M0, &nM);
     if (rc) return rc;
+    ForkJoin fj(p, nv > 0 && ng > 1);  // scalar groups beside the vector group
     for (int i = 0; i < ng; i++) {
+      const hipStream_t sm = fj.stream(!g[i].vec);
       SplitDims d = split_dims(p, ncomp, (ne_max + 63) / 64);
       d.ncg = g[i].ncg;
       d.M0 = M0;
@@ -1792,11 +1798,11 @@ static int fwd_split_S(sht_plan *p, const Workspace &ws, int nv, int ns, const d
 #define KSHT_FWD(NB, VEC)                                                                      \
   if (p->var_fwd_mfma == 32)                                                                   \
     hipLaunchKernelGGL((k_leg_fwd_split32<S, NB, VEC>), grid, dim3(kGemm32Threads),            \
-                       fwd32_lds<S>(NB), p->stream, d, g[i].nd_first, tb, ws.dig, ws.dexp, sp, \
+                       fwd32_lds<S>(NB), sm, d, g[i].nd_first, tb, ws.dig, ws.dexp, sp,        \
                        sa, p->t.jst, p->t.ipad, p->t.radius);                                  \
   else                                                                                         \
     hipLaunchKernelGGL((k_leg_fwd_split<S, NB, VEC>), grid, dim3(kGemmThreads),                \
-                       fwd_lds<S>(NB), p->stream, d, g[i].nd_first, tb, ws.dig, ws.dexp, sp,   \
+                       fwd_lds<S>(NB), sm, d, g[i].nd_first, tb, ws.dig, ws.dexp, sp,          \
                        sa, p->t.jst, p->t.ipad, p->t.radius)
       if (g[i].vec) {
         StageTimer tk(p, 6);

@@ -182,9 +182,9 @@ extern "C" int sht_plan_autotune(sht_plan *p, int nvb, int nsb, int nvf, int nsf
         if ((rc = trip())) break;
         double ms_best = 1e300;
         for (int rep = 0; rep < 3 && !rc; rep++) {
-          (void)hipEventRecord(a, p->stream);
+          (void)hipEventRecord(a, api_stream(p->stream));
           rc = trip();
-          (void)hipEventRecord(b, p->stream);
+          (void)hipEventRecord(b, api_stream(p->stream));
           if (hipEventSynchronize(b) != hipSuccess) {
             fail(SHT_ERR_HIP, "timing run failed");
             break;
@@ -206,7 +206,7 @@ extern "C" int sht_plan_autotune(sht_plan *p, int nvb, int nsb, int nvf, int nsf
   }
   if (a) (void)hipEventDestroy(a);
   if (b) (void)hipEventDestroy(b);
-  (void)hipStreamSynchronize(p->stream);
+  (void)hipStreamSynchronize(api_stream(p->stream));
   if (sp) (void)hipFree(sp);
   if (grid) (void)hipFree(grid);
   if (rc) {

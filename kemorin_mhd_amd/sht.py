@@ -623,6 +623,7 @@ class ShellBatch:
         if getattr(self, "_h", None) and self._h.value:
             lib().sht_batch_destroy(self._h)
             self._h = _vp()
+        self._keep = []  # the plans and buffers the graph used
 
     def __del__(self):
         try:
