@@ -1736,11 +1736,12 @@ static int bwd_split_S(sht_plan *p, const Workspace &ws, int nv, int ns, const d
     int M0 = 0, nM = 0;
     int rc = split_tile<S>(p, st, ti, false, &tb, &M0, &nM);
     if (rc) return rc;
-    // the scalar groups run beside the vector group (their work-groups fill
-    // the tail of the vector launch)
-    ForkJoin fj(p, nv > 0 && ng > 1);
+    // the groups run in order: beside the vector group (ForkJoin, as the
+    // FP64 kernels do) the scalar group only slowed it by as much as it
+    // saved (15.5 against 15.3 ms at L=511), the split GEMMs keeping the
+    // chip busy to their last work-groups
+    const hipStream_t sm = p->stream;
     for (int i = 0; i < ng; i++) {
-      const hipStream_t sm = fj.stream(!g[i].vec);
       SplitDims d = split_dims(p, ncomp, p->nth_hp / 64);
       d.ncg = g[i].ncg;
       d.M0 = M0;
@@ -1784,9 +1785,8 @@ static int fwd_split_S(sht_plan *p, const Workspace &ws, int nv, int ns, const d
     int M0 = 0, nM = 0;
     int rc = split_tile<S>(p, st, ti, true, &tb, &M0, &nM);
     if (rc) return rc;
-    ForkJoin fj(p, nv > 0 && ng > 1);  // scalar groups beside the vector group
+    const hipStream_t sm = p->stream;  // groups in order, as in the backward
     for (int i = 0; i < ng; i++) {
-      const hipStream_t sm = fj.stream(!g[i].vec);
       SplitDims d = split_dims(p, ncomp, (ne_max + 63) / 64);
       d.ncg = g[i].ncg;
       d.M0 = M0;
