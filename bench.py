@@ -430,25 +430,66 @@ def kernel_roofline(cfg_d, res, precision, slices):
             "work_per_launch": work, "avg_ms_per_launch": avg[dom]}, per_stage
 
 
-def cpu_baseline(cfg_d, nr_sample):
+def sample_radii(cfg_d, nr_sample):
+    """The radius sample of the CPU baseline (both boundaries included)."""
+    nr_s = min(nr_sample, cfg_d["nr"])
+    return np.linspace(0, cfg_d["nr"] - 1, nr_s).round().astype(int)
+
+
+def gather_sample(cfg, ks, sp_b, grid_f, results):
+    """Host copies, on the radius sample ks, of the benchmark inputs and of
+    each engine's outputs (the oracle's layouts at len(ks) radii)."""
+    def grid(v, nc):
+        return v.view(nc, cfg.nphi, cfg.nth, cfg.nr)[..., ks].contiguous().cpu().numpy().ravel()
+
+    def spec(s, nc):
+        return s.view(cfg.nr, cfg.jmax, nc)[ks].contiguous().cpu().numpy().ravel()
+
+    out = {p: (grid(r["outputs"][0], 8), spec(r["outputs"][1], 9)) for p, r in results.items()}
+    return {"ks": ks, "sp_b": spec(sp_b, 8), "grid_f": grid(grid_f, 9), "outputs": out}
+
+
+def oracle_errors(ref_b, ref_f, got_b, got_f):
+    """Per-field relative L2 and max errors of an engine against the oracle."""
+    def err(a, b):
+        return {"rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+                "rel_max": float(np.abs(a - b).max() / np.abs(b).max())}
+    gb, rb = got_b.reshape(8, -1), ref_b.reshape(8, -1)
+    gf, rf = got_f.reshape(-1, 9), ref_f.reshape(-1, 9)
+    bwd = {"vector1": err(gb[0:3], rb[0:3]), "vector2": err(gb[3:6], rb[3:6]),
+           "scalar1": err(gb[6], rb[6]), "scalar2": err(gb[7], rb[7])}
+    fwd = {f"vector{i + 1}": err(gf[:, 3 * i:3 * i + 3], rf[:, 3 * i:3 * i + 3]) for i in range(3)}
+    return {"backward": bwd, "forward": fwd}
+
+
+def cpu_baseline(cfg_d, nr_sample, sample=None):
+    """The oracle timed on a radius sample of the workload; with `sample`
+    (the benchmark's own inputs and the engines' outputs on those radii) it
+    also reports each engine's error against the oracle there."""
     import oracle as O
     from kemorin_mhd_amd import chebyshev_radius
-    nr_s = min(nr_sample, cfg_d["nr"])
-    r_full = chebyshev_radius(cfg_d["nr"], 0.35 / 0.65, 1 / 0.65)
-    r = r_full[np.linspace(0, cfg_d["nr"] - 1, nr_s).round().astype(int)]
+    ks = sample_radii(cfg_d, nr_sample)
+    nr_s = len(ks)
+    r = chebyshev_radius(cfg_d["nr"], 0.35 / 0.65, 1 / 0.65)[ks]
     P = O.OraclePlan(cfg_d["L"], nr_s, cfg_d["nth"], cfg_d["nphi"], r, full_tables=False)
-    rng = np.random.default_rng(1)
-    sp_b = rng.standard_normal(P.n_rj(8))
-    grid_f = rng.standard_normal(P.n_rtp(9))
+    if sample is not None:
+        sp_b, grid_f = sample["sp_b"], sample["grid_f"]
+    else:
+        rng = np.random.default_rng(1)
+        sp_b = rng.standard_normal(P.n_rj(8))
+        grid_f = rng.standard_normal(P.n_rtp(9))
     times = []
     for _ in range(3):
         t0 = time.perf_counter()
-        P.backward(2, 2, sp_b)
-        P.forward(3, 0, grid_f)
+        ref_b = P.backward(2, 2, sp_b)
+        ref_f = P.forward(3, 0, grid_f)
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
     full = t * cfg_d["nr"] / nr_s
-    return {"value": 1.0 / full, "unit": "round_trips/s", "cores": O.lib().orc_num_threads(),
+    errors = None
+    if sample is not None:
+        errors = {p: oracle_errors(ref_b, ref_f, *o) for p, o in sample["outputs"].items()}
+    return errors, {"value": 1.0 / full, "unit": "round_trips/s", "cores": O.lib().orc_num_threads(),
             "kind": "port",
             "sample": (f"L={cfg_d['L']} Ntheta={cfg_d['nth']} Nphi={cfg_d['nphi']}, {nr_s} of "
                        f"{cfg_d['nr']} radii, median of 3 round trips ({t:.2f} s each), "
@@ -511,6 +552,9 @@ def main():
                                 alias=alias)
         print(f"# {p}: {results[p]['ms_per_step']:.3f} ms/round trip", file=sys.stderr, flush=True)
     accuracy = None
+    sample = None
+    if keep and not args.no_cpu:
+        sample = gather_sample(cfg, sample_radii(cfg_d, args.cpu_radii), sp_b, grid_f, results)
     if keep:
         del sp_b, grid_f
         accuracy = split_accuracy(cfg, args, torch, results)
@@ -548,7 +592,10 @@ def main():
         out["speedup_split_vs_fp64"] = results["fp64"]["ms_per_step"] / results["split"]["ms_per_step"]
         out["split_error_vs_fp64"] = accuracy
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg_d, args.cpu_radii)
+        errors, out["cpu_baseline"] = cpu_baseline(cfg_d, args.cpu_radii, sample)
+        if errors:
+            # each engine against the oracle on the CPU baseline's radius sample
+            out["error_vs_oracle"] = {"radii": [int(k) for k in sample["ks"]], **errors}
     print(json.dumps(out))
 
 
