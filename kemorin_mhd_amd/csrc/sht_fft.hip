@@ -487,7 +487,10 @@ struct TwoLevelRoots {
 };
 
 // tw: roots of unity in LDS, of order NP at stride ts (the NP-entry table
-// with ts = 1, or the N-entry table with ts = R0)
+// with ts = 1, or the N-entry table with ts = R0).  A butterfly's twiddles
+// w^r, r < R, are powers of its one root w: one table read and R - 2 complex
+// products instead of R - 1 reads, whose strides (k r entries apart across
+// the lanes) made the LDS serve them in two to four passes.
 template <int NP, int Ns>
 __device__ __forceinline__ void f3_passes(double2 (&v)[8], double2 *ln, int j,
                                           const double2 *__restrict__ tw, int ts, double s) {
@@ -498,12 +501,13 @@ __device__ __forceinline__ void f3_passes(double2 (&v)[8], double2 *ln, int j,
     const int b = j + T * i;
     if constexpr (Ns > 1) {
       const int k = b % Ns;
+      double2 w1 = tw[k * (NP / (Ns * R)) * ts];  // exp(-2 pi i k / (Ns R))
+      w1.y *= -s;
+      double2 w = w1;
 #pragma unroll
       for (int r = 1; r < R; r++) {
-        // exp(-2 pi i k r / (Ns R)) = tw[k r (NP / (Ns R)) ts]; k r < Ns R
-        double2 w = tw[k * r * (NP / (Ns * R)) * ts];
-        w.y *= -s;
         v[i * R + r] = cmul(v[i * R + r], w);
+        if (r + 1 < R) w = cmul(w, w1);
       }
     }
     dft_small<R>(v + i * R, s);
@@ -582,14 +586,6 @@ template <int NP>
 __device__ __forceinline__ constexpr int f3_pos(int q) {
   constexpr int R = f3_last_radix<NP>();
   return (q % (8 / R)) * R + q / (8 / R);
-}
-
-// twiddle indices of the R0 aliases: m j1 and (m + N - NP) j1 modulo N, for
-// m < NP, j1 < R0 (N = R0 NP): the first is below N, the second equals
-// m j1 + N - NP j1 for j1 >= 1
-__device__ __forceinline__ int alias_lo(int m, int j1) { return m * j1; }
-__device__ __forceinline__ int alias_hi(int m, int j1, int N, int NP) {
-  return j1 == 0 ? 0 : m * j1 + N - NP * j1;
 }
 
 // exp(-2 pi i k / N), k < N = R0 NP, as exp(-2 pi i a / NP) exp(-2 pi i b / N)
@@ -735,14 +731,19 @@ __global__ void __launch_bounds__(NC * NP / 8, 2) k_fft3_bwd(int N, int R0, unsi
   const bool vec2 = (nline & 1) == 0;
   for (int j1 = 0; j1 < R0; j1++) {
     double2 w[8];
+    // e^{+2 pi i m j1 / N} for m = j + T q as e^{+2 pi i j j1 / N} times the
+    // q-th power of e^{+2 pi i T j1 / N}, and the alias m + N - NP one factor
+    // e^{-2 pi i NP j1 / N} further
+    double2 t1 = tw_n(twp, twr, j * j1, R0, r0m);
+    double2 st = tw_n(twp, twr, T * j1, R0, r0m);
+    double2 hf = j1 == 0 ? make_double2(1.0, 0.0) : tw_n(twp, twr, N - NP * j1, R0, r0m);
+    t1.y = -t1.y;  // e^{+...}
+    st.y = -st.y;
+    hf.y = -hf.y;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      const int m = j + T * q;
-      double2 t1 = tw_n(twp, twr, alias_lo(m, j1), R0, r0m);
-      double2 t2 = tw_n(twp, twr, alias_hi(m, j1, N, NP), R0, r0m);
-      t1.y = -t1.y;  // e^{+...}
-      t2.y = -t2.y;
-      w[q] = cfma(zhi[q], t2, cmul(zlo[q], t1));
+      w[q] = cfma(zhi[q], cmul(t1, hf), cmul(zlo[q], t1));
+      if (q < 7) t1 = cmul(t1, st);
     }
     f3_passes<NP, 1>(w, ln, j, twp, 1, 1.0);
     __syncthreads();  // every wave done with its pass area (under the staging)
@@ -767,6 +768,28 @@ __global__ void __launch_bounds__(NC * NP / 8, 2) k_fft3_bwd(int N, int R0, unsi
       }
     }
     __syncthreads();
+  }
+}
+
+// Forward accumulation of decimation j1: e^{-2 pi i k j1 / N} for k = j + T q
+// as e^{-2 pi i j j1 / N} times the q-th power of e^{-2 pi i T j1 / N} (three
+// table reads per thread instead of sixteen), the alias k + N - NP one factor
+// e^{-2 pi i (N - NP j1) / N} further.  twl: exp(-2 pi i k / N), k < N.
+template <int NP>
+__device__ __forceinline__ void f3_fwd_alias(const double2 (&w)[8], double2 (&zlo)[8],
+                                             double2 (&zhi)[8], const double2 *twl, int j,
+                                             int j1, int N, int L) {
+  constexpr int T = NP / 8;
+  double2 t = twl[j * j1];
+  const double2 st = twl[T * j1];
+  const double2 hf = j1 == 0 ? make_double2(1.0, 0.0) : twl[N - NP * j1];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int k = j + T * q;
+    const double2 y = w[f3_pos<NP>(q)];
+    if (k <= L) zlo[q] = cfma(y, t, zlo[q]);
+    if (k >= NP - L) zhi[q] = cfma(y, cmul(t, hf), zhi[q]);
+    if (q < 7) t = cmul(t, st);
   }
 }
 
@@ -824,14 +847,7 @@ __global__ void __launch_bounds__(NC * NP / 8) k_fft3_fwd(int N, int R0, int L,
       __syncthreads();
       if (j1 + 1 < R0) issue(j1 + 1);
       f3_passes<NP, 1>(w, ln, j, twl, R0, -1.0);
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const int k = j + T * q;
-        const double2 y = w[f3_pos<NP>(q)];
-        const double2 tlo = twl[alias_lo(k, j1)], thi = twl[alias_hi(k, j1, N, NP)];
-        if (k <= L) zlo[q] = cfma(y, tlo, zlo[q]);
-        if (k >= NP - L) zhi[q] = cfma(y, thi, zhi[q]);
-      }
+      f3_fwd_alias<NP>(w, zlo, zhi, twl, j, j1, N, L);
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   } else
@@ -869,14 +885,7 @@ __global__ void __launch_bounds__(NC * NP / 8) k_fft3_fwd(int N, int R0, int L,
     for (int q = 0; q < 8; q++) w[q] = lds[(j + T * q) * (NC + 1) + cl];
     __syncthreads();
     f3_passes<NP, 1>(w, ln, j, twl, R0, -1.0);
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int k = j + T * q;
-      const double2 y = w[f3_pos<NP>(q)];
-      const double2 tlo = twl[alias_lo(k, j1)], thi = twl[alias_hi(k, j1, N, NP)];
-      if (k <= L) zlo[q] = cfma(y, tlo, zlo[q]);
-      if (k >= NP - L) zhi[q] = cfma(y, thi, zhi[q]);
-    }
+    f3_fwd_alias<NP>(w, zlo, zhi, twl, j, j1, N, L);
   }
   // Z_{N-m} for m = j + T q sits at index NP - m of the lane (T - j) % T of
   // this complex line, in register 7 - q (or (8 - q) % 8 when j = 0)
