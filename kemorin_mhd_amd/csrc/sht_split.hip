@@ -261,12 +261,18 @@ constexpr int kSliceThreads = 256;
 // of each vector (pol) is scaled by g3 = l(l+1) (1/2 at l = 0) as in the FP64
 // kernels; radii nr <= k < nr_p get zero digits.
 constexpr int kBsRadii = 4;  // lanes per wave: 4 radii x 16
+// Staging offset of element e of radius rr's run: one double of padding per
+// 32, so that the slicing lanes (8 ncomp doubles apart, 512 bytes for 8
+// components) fall into different LDS banks instead of one.
+__device__ __forceinline__ int bs_at(int rr, int e, int ncomp) {
+  return rr * 132 * ncomp + e + (e >> 5);
+}
 template <int S>
 __global__ void __launch_bounds__(kSliceThreads) k_bslice(
     int L, int nr, int nr_p, int nrad, int nv, int ncomp, const double *__restrict__ sp, SpecAddr sa,
     const int *__restrict__ jst, const long long *__restrict__ cb, signed char *__restrict__ dig,
     int *__restrict__ dexp) {
-  extern __shared__ double run[];  // [nrad][128 degrees][ncomp]
+  extern __shared__ double run[];  // [nrad][128 degrees][ncomp], padded (bs_at)
   const int kq = blockIdx.x, sl = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int M = sl >= L ? sl - L : L - sl;
@@ -305,7 +311,7 @@ __global__ void __launch_bounds__(kSliceThreads) k_bslice(
 #pragma unroll
         for (int u = 0; u < 4; u++) {
           const int e = tid + u * kSliceThreads;
-          if (rr < nrad && e < nd_run) run[rr * 128 * ncomp + e] = live ? val[rr][u] : 0.0;
+          if (rr < nrad && e < nd_run) run[bs_at(rr, e, ncomp)] = live ? val[rr][u] : 0.0;
         }
       }
     } else if (sa.s_nd == 1 && sa.s_j == ncomp) {
@@ -315,7 +321,6 @@ __global__ void __launch_bounds__(kSliceThreads) k_bslice(
       for (int rr = 0; rr < nrad; rr++) {
         const int kr = min(k0 + rr, nr - 1);  // radii >= nr read radius nr - 1
         const double *rs = src + (long long)kr * sa.s_k + (long long)d0 * ncomp;
-        double *rd = run + rr * 128 * ncomp;
         const bool live = k0 + rr < nr;
         for (int base = 0; base < nd_run; base += 8 * kSliceThreads) {
           double val[8];
@@ -327,7 +332,7 @@ __global__ void __launch_bounds__(kSliceThreads) k_bslice(
 #pragma unroll
           for (int u = 0; u < 8; u++) {
             const int e = base + tid + u * kSliceThreads;
-            if (e < nd_run) rd[e] = live ? val[u] : 0.0;
+            if (e < nd_run) run[bs_at(rr, e, ncomp)] = live ? val[u] : 0.0;
           }
         }
       }
@@ -351,7 +356,7 @@ __global__ void __launch_bounds__(kSliceThreads) k_bslice(
         const int idx = base + tid + u * kSliceThreads;
         if (idx < nall) {
           const int rr = idx / nd_run, e = idx - rr * nd_run;
-          run[rr * 128 * ncomp + e] = k0 + rr < nr ? val[u] : 0.0;
+          run[bs_at(rr, e, ncomp)] = k0 + rr < nr ? val[u] : 0.0;
         }
       }
     }
@@ -367,7 +372,7 @@ __global__ void __launch_bounds__(kSliceThreads) k_bslice(
         const int il = 4 * q16 + j, i = i0 + il;
         x[j] = 0.0;
         if (r4 < nrad && il < ni && i < npar) {
-          x[j] = run[r4 * 128 * ncomp + (2 * il + par) * ncomp + cq];
+          x[j] = run[bs_at(r4, (2 * il + par) * ncomp + cq, ncomp)];
           if (cq < 3 * nv && cq % 3 == 0) x[j] *= g3s_of(M + 2 * i + par);
         }
         e = max(e, exp_of(x[j]));
@@ -1629,6 +1634,8 @@ static int slice_tables(sht_plan *p, SplitState *st) {
   KSHT_LDS((k_leg_fwd_split32<S, 3, false>), fwd32_lds<S>(3));
   KSHT_LDS((k_leg_fwd_split32<S, 2, false>), fwd32_lds<S>(2));
   KSHT_LDS((k_leg_fwd_split32<S, 1, false>), fwd32_lds<S>(1));
+  // one radius of 64 components: 132 * 64 doubles of padded staging
+  KSHT_LDS(k_bslice<S>, (int)sizeof(double) * 132 * 64);
 #undef KSHT_LDS
   return SHT_OK;
 }
@@ -1721,11 +1728,11 @@ static int bwd_split_S(sht_plan *p, const Workspace &ws, int nv, int ns, const d
                        SpecAddr sa, double *out, GridAddr ga) {
   SplitState *st = (SplitState *)p->split;
   const int ncomp = 3 * nv + ns;
-  // radii per work-group: 4, unless the staging [nrad][128][ncomp] doubles
-  // would pass 64 KiB of LDS (nr_p is a multiple of 16)
-  const int nrad = ncomp <= 16 ? 4 : ncomp <= 32 ? 2 : 1;
+  // radii per work-group: 4, unless the padded staging (132 ncomp doubles
+  // per radius) would pass 64 KiB of LDS (nr_p is a multiple of 16)
+  const int nrad = ncomp <= 15 ? 4 : ncomp <= 31 ? 2 : 1;
   hipLaunchKernelGGL(k_bslice<S>, dim3(p->nr_p / nrad, p->nslot), dim3(kSliceThreads),
-                     sizeof(double) * nrad * 128 * ncomp, p->stream,
+                     sizeof(double) * nrad * 132 * ncomp, p->stream,
                      p->L, p->nr, p->nr_p, nrad, nv, ncomp, spec, sa, p->t.jst, st->cb, ws.dig,
                      ws.dexp);
   KSHT_HIP(hipGetLastError());
