@@ -51,6 +51,14 @@ struct LegDims {
 // registers while the MFMAs of chunk ic run on the other LDS buffer.
 constexpr int kSclPer = 4;
 
+// Column of (radius kk) within the 16-double row (ii, par) of the backward's
+// staging: XOR-swizzled by 3 bits of ii and the parity, so that the stores
+// (lanes over 16 ii x 2 parities at one radius) reach 32 different banks;
+// the MFMA reads (16 radii of one row per 16 lanes) still cover whole rows.
+__device__ __forceinline__ int bwd_col(int ii, int par, int kk) {
+  return kk ^ (((ii >> 1) & 7) | (par << 3));
+}
+
 template <bool VEC>
 __global__ void __launch_bounds__(256) k_leg_bwd(
     LegDims d, int cg0, const double *__restrict__ sp, SpecAddr sa, double *__restrict__ out,
@@ -120,7 +128,8 @@ __global__ void __launch_bounds__(256) k_leg_bwd(
       const int lo = idx & 31, kk = (idx >> 5) & 15, sg = idx >> 9;
       const int par = lo & 1, ii = lo >> 1;
 #pragma unroll
-      for (int c = 0; c < NC; c++) buf[(((sg * NC + c) * 2 + par) * 16 + ii) * 16 + kk] = sreg[u][c];
+      for (int c = 0; c < NC; c++)
+        buf[(((sg * NC + c) * 2 + par) * 16 + ii) * 16 + bwd_col(ii, par, kk)] = sreg[u][c];
     }
   };
   double pc[4][2], dpc[4][2], pn[4][2], dpn[4][2];
@@ -161,7 +170,8 @@ __global__ void __launch_bounds__(256) k_leg_bwd(
 #pragma unroll
           for (int sg = 0; sg < 2; sg++) {
             if (sg >= nsig) continue;
-            const double *bb = buf + ((sg * NC * 2 + par) * 16 + ii) * 16 + (lane & 15);
+            const double *bb =
+                buf + ((sg * NC * 2 + par) * 16 + ii) * 16 + bwd_col(ii, par, lane & 15);
             if (VEC) {
               const double bpol = bb[0], bdpol = bb[512], btor = bb[1024];
               acc[sg][par][0] = mfma64(aP, bpol, acc[sg][par][0]);
