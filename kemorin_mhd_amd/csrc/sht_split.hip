@@ -95,19 +95,30 @@ __device__ __forceinline__ int exp_of(double v) {
 // d_k = u_k - 128 in [-128, 127] for k >= 1 and d_0 = (F + bias) >> 8(S-1) in
 // [-64, 64].  Then v ~= 2^(e+2) sum_k d_k 2^-8(k+1) with error <= 2^(e+1-8S),
 // unbiased (round to nearest), and every digit fits in an int8.
+// For S <= 6 the shift is a right shift for every non-zero v (sh <= 8S - 55
+// < 0), and a zero mantissa rounds to zero for any right shift: no branches.
 template <int S>
 __device__ __forceinline__ long long fix_of(double v, int e) {
   constexpr long long kBias = 128LL * (((1LL << (8 * (S - 1))) - 1) / 255);
   const long long bits = __double_as_longlong(v);
   const int ex = (int)((bits >> 52) & 0x7ff);
-  unsigned long long mag = 0;
-  if (ex != 0) {
-    const unsigned long long mant = (unsigned long long)(bits & 0xFFFFFFFFFFFFFll) | (1ull << 52);
-    const int sh = ex - 1077 - e + 8 * S;  // mag = mant 2^sh; sh <= 8S - 55 when |v| < 2^e
-    if (sh >= 0)
-      mag = sh < 10 ? (mant << sh) : 0;
-    else if (sh > -64)
-      mag = (mant + (1ull << (-sh - 1))) >> (-sh);
+  const int sh = ex - 1077 - e + 8 * S;  // mag = mant 2^sh; sh <= 8S - 55 when |v| < 2^e
+  unsigned long long mag;
+  if constexpr (8 * S - 55 < 0) {
+    const unsigned long long mant =
+        ex != 0 ? ((unsigned long long)(bits & 0xFFFFFFFFFFFFFll) | (1ull << 52)) : 0ull;
+    const int rs = min(max(-sh, 1), 63);
+    mag = (mant + (1ull << (rs - 1))) >> rs;
+  } else {
+    mag = 0;
+    if (ex != 0) {
+      const unsigned long long mant =
+          (unsigned long long)(bits & 0xFFFFFFFFFFFFFll) | (1ull << 52);
+      if (sh >= 0)
+        mag = sh < 10 ? (mant << sh) : 0;
+      else if (sh > -64)
+        mag = (mant + (1ull << (-sh - 1))) >> (-sh);
+    }
   }
   const long long F = bits < 0 ? -(long long)mag : (long long)mag;
   return F + kBias;
@@ -120,10 +131,18 @@ __device__ __forceinline__ unsigned digit_byte(long long Fb, int k) {
   return (unsigned)(((int)((Fb >> (8 * (S - 1 - k))) & 255)) - 128) & 0xffu;
 }
 
+// digit k of four values, one byte each: the byte of the 32-bit half that
+// holds it (u_k - 128 = u_k ^ 0x80 as a byte for k >= 1)
 template <int S>
 __device__ __forceinline__ unsigned pack4(const long long *Fb, int k) {
-  return digit_byte<S>(Fb[0], k) | (digit_byte<S>(Fb[1], k) << 8) |
-         (digit_byte<S>(Fb[2], k) << 16) | (digit_byte<S>(Fb[3], k) << 24);
+  const int bit = 8 * (S - 1 - k);
+  unsigned r = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const unsigned w = (unsigned)((unsigned long long)Fb[i] >> (bit & ~31));
+    r |= ((w >> (bit & 31)) & 0xffu) << (8 * i);
+  }
+  return k == 0 ? r : r ^ 0x80808080u;
 }
 
 // ---------------------------------------------------------------------------
@@ -426,26 +445,34 @@ __global__ void __launch_bounds__(kSliceThreads) k_fslice(
   const bool k_fast = ga.s_k <= ga.s_t;
   const long long base = ga.slot_off[sl] * ga.slot_mul;
   const long long sstride = (long long)ntile * 1024;
-  for (int c = 0; c < nchf; c++) {
-    for (int cg = 0; cg < ncomp; cg += kFsGroup) {
-      const int ng = min(kFsGroup, ncomp - cg);
-      __syncthreads();
-      double vn[kFsGroup * 4], vs[kFsGroup * 4];
+  // (chunk, component group) steps; the rows of step it + 1 are loaded while
+  // step it is sliced (its rows are in LDS by then)
+  const int ncg = (ncomp + kFsGroup - 1) / kFsGroup, nit = nchf * ncg;
+  double vn[kFsGroup * 4], vs[kFsGroup * 4];
+  auto load_rows = [&](int it) {
+    const int c = it / ncg, cg = (it % ncg) * kFsGroup;
+    const int ng = min(kFsGroup, ncomp - cg);
 #pragma unroll
-      for (int u = 0; u < kFsGroup * 4; u++) {
-        const int f = u / 4, idx = tid + kSliceThreads * (u % 4);
-        const int kk = k_fast ? (idx & 15) : (idx >> 6);
-        const int tt = k_fast ? (idx >> 4) : (idx & 63);
-        const int t = c * 64 + tt, k = kt * 16 + kk;
-        // unconditional loads (clamped into range, zeroed after) so that all
-        // of them stay in flight together
-        // (radii past the last read the last one: the line the tile reads anyway)
-        const bool ok = f < ng && k < nr && t < nth_h;
-        const int fc = f < ng ? f : 0, kc = k < nr ? k : nr - 1, tc = t < nth_h ? t : 0;
-        const long long b = base + (long long)(cg + fc) * ga.s_nd + (long long)kc * ga.s_k;
-        vn[u] = in[b + (long long)tc * ga.s_t];
-        vs[u] = in[b + (long long)(nth - 1 - tc) * ga.s_t];
-      }
+    for (int u = 0; u < kFsGroup * 4; u++) {
+      const int f = u / 4, idx = tid + kSliceThreads * (u % 4);
+      const int kk = k_fast ? (idx & 15) : (idx >> 6);
+      const int tt = k_fast ? (idx >> 4) : (idx & 63);
+      const int t = c * 64 + tt, k = kt * 16 + kk;
+      // unconditional loads (clamped into range, zeroed at the LDS store) so
+      // that all of them stay in flight together (radii past the last read
+      // the last one: the line the tile reads anyway)
+      const int fc = f < ng ? f : 0, kc = k < nr ? k : nr - 1, tc = t < nth_h ? t : 0;
+      const long long b = base + (long long)(cg + fc) * ga.s_nd + (long long)kc * ga.s_k;
+      vn[u] = in[b + (long long)tc * ga.s_t];
+      vs[u] = in[b + (long long)(nth - 1 - tc) * ga.s_t];
+    }
+  };
+  load_rows(0);
+  for (int it = 0; it < nit; it++) {
+    {
+      const int c = it / ncg, cg = (it % ncg) * kFsGroup;
+      const int ng = min(kFsGroup, ncomp - cg);
+      __syncthreads();  // the previous step's slicing done with sC
 #pragma unroll
       for (int u = 0; u < kFsGroup * 4; u++) {
         const int f = u / 4, idx = tid + kSliceThreads * (u % 4);
@@ -459,6 +486,7 @@ __global__ void __launch_bounds__(kSliceThreads) k_fslice(
         sC[f][1][tt][kk] = (xn - xs) * w;
       }
       __syncthreads();
+      if (it + 1 < nit) load_rows(it + 1);
       // wave task = (component, sym/asym) block
       for (int task = wave; task < ng * 2; task += kSliceThreads / 64) {
         const int h = task & 1, f = task >> 1;
