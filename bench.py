@@ -292,6 +292,10 @@ def run_engine(cfg, precision, args, torch, sp_b, grid_f, keep=False, otf=False,
     timers = plan.timer_read()
     plan.timer_enable(False)
     dev_bytes = plan.device_bytes()
+    # the split backward's vector launches also take one scalar each (5 or 6
+    # slices, the default variant): 2 x (5 + 1) contractions in one kernel
+    fused = (precision == "split" and (args.slices or 6) <= 6
+             and plan.get_variant("split_bwd_fused") == 1)
     plan.close()
     out = (grid_b, sp_f) if keep else None
     if not keep:
@@ -300,7 +304,8 @@ def run_engine(cfg, precision, args, torch, sp_b, grid_f, keep=False, otf=False,
     ms = (t1 - t0) * 1e3 / args.steps
     return dict(ms_per_step=ms, round_trips_per_s=1e3 / ms, timers=timers, plan_bytes=dev_bytes,
                 outputs=out, slices=(args.slices or 6) if precision == "split" else 0,
-                hbm_in_use_bytes=total_b - free_b, hbm_total_bytes=total_b, autotune=tuned)
+                hbm_in_use_bytes=total_b - free_b, hbm_total_bytes=total_b, autotune=tuned,
+                bwd_fused=fused)
 
 
 FIELDS_BWD = (("vector1", 0, 3), ("vector2", 3, 6), ("scalar1", 6, 7), ("scalar2", 7, 8))
@@ -371,7 +376,8 @@ def kernel_roofline(cfg_d, res, precision, slices):
     """Roofline of the round trip's dominant kernel.  Legendre kernels: the
     vector-group kernel alone (stage timers 5/6, HIP events on the plan's
     stream), algorithmic work = FP64 flops of its contractions (10 F1 backward:
-    2 vectors x 5, 15 F1 forward: 3 vectors x 5); in the split engine each FP64
+    2 vectors x 5, or 12 F1 where the split backward's vector launch also
+    takes the two scalars; 15 F1 forward: 3 vectors x 5); in the split engine each FP64
     multiply-add is S(S+1)/2 INT8 slice products, so its INT8 work is that
     many times the FP64 flops, against the INT8 MFMA peak.  FFT stages: the
     bytes a launch must move (X in, grid out, or the reverse)."""
@@ -388,15 +394,16 @@ def kernel_roofline(cfg_d, res, precision, slices):
     fast = npow <= 512 and cfg_d["nphi"] % npow == 0
     split = precision == "split"
     kname = {
-        "legendre_backward_vector_kernel": (f"ksht::k_leg_bwd_split<{slices}, 3, true>" if split
-                                            else "ksht::k_leg_bwd<true>"),
+        "legendre_backward_vector_kernel": (
+            f"ksht::k_leg_bwd_split<{slices}, {4 if res.get('bwd_fused') else 3}, true>" if split
+            else "ksht::k_leg_bwd<true>"),
         "legendre_forward_vector_kernel": (f"ksht::k_leg_fwd_split<{slices}, 3, true>" if split
                                            else "ksht::k_leg_fwd<true>"),
         "fft_backward": f"ksht::k_fft3_bwd<{npow}, 4>" if fast else "ksht::k_fft_bwd (mixed radix)",
         "fft_forward": f"ksht::k_fft3_fwd<{npow}, 8>" if fast else "ksht::k_fft_fwd (mixed radix)",
     }
     stages = {
-        "legendre_backward_vector_kernel": ("mfma", 10 * F1),
+        "legendre_backward_vector_kernel": ("mfma", (12 if res.get("bwd_fused") else 10) * F1),
         "legendre_forward_vector_kernel": ("mfma", 15 * F1),
         "fft_backward": ("hbm", 8.0 * 8 * nline * (nslot + cfg_d["nphi"])),
         "fft_forward": ("hbm", 8.0 * 9 * nline * (nslot + cfg_d["nphi"])),

@@ -92,6 +92,11 @@ long long sht_plan_device_bytes(const sht_plan *plan);
  *   "split_fwd_mfma": 16 or 32, the INT8 MFMA form of the forward split
  *                     Legendre products (v_mfma_i32_16x16x64_i8 or
  *                     v_mfma_i32_32x32x32_i8);
+ *   "split_bwd_fused": 1 (default) or 0: in the backward split products,
+ *                     each vector's launch also takes one scalar field,
+ *                     whose product shares the vector's P slices (1, for
+ *                     5 or 6 slices), or the scalars run in launches of
+ *                     their own (0); the results are the same bit for bit;
  *   "fft_form":       1, 0, 2 or 3: the split-form FFT (Nphi = r0 2^k,
  *                     2^k > L; default where the length has it), the
  *                     mixed-radix Stockham FFT, the pruned real DFT as one

@@ -152,6 +152,9 @@ struct sht_plan {
   // kernel variants (sht_plan_set_variant): INT8 MFMA form of the forward
   // split GEMM, 16 (16x16x64) or 32 (32x32x32)
   int var_fwd_mfma = 16;
+  // split backward: 1 = each vector group's launch also takes one scalar
+  // (sharing its P slices), 0 = scalars in launches of their own
+  int var_bwd_fused = 1;
   // FFT form: 1 = split form n = r0 * 2^k where the plan has it, 0 = the
   // mixed-radix Stockham form for every length, 2 = the pruned real DFT as an
   // FP64 GEMM (set-up on first selection: sht_dft.hip)

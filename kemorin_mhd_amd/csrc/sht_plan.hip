@@ -533,11 +533,16 @@ extern "C" long long sht_plan_device_bytes(const sht_plan *p) { return p ? p->de
 
 // Kernel variants: name -> the plan field and its allowed values.
 static int *variant_slot(sht_plan *p, const char *name, const int **allowed, int *nallowed) {
-  static const int mfma[] = {16, 32}, form[] = {0, 1, 2, 3};
+  static const int mfma[] = {16, 32}, form[] = {0, 1, 2, 3}, onoff[] = {0, 1};
   if (!strcmp(name, "split_fwd_mfma")) {
     *allowed = mfma;
     *nallowed = 2;
     return &p->var_fwd_mfma;
+  }
+  if (!strcmp(name, "split_bwd_fused")) {
+    *allowed = onoff;
+    *nallowed = 2;
+    return &p->var_bwd_fused;
   }
   if (!strcmp(name, "fft_form")) {
     *allowed = form;

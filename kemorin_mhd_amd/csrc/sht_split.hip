@@ -668,6 +668,10 @@ __device__ __forceinline__ double pair_scale(int g0, bool two) {
 //   odd l-M:  P terms N and -S, dP terms both,
 // and per output slot: vr = a^2 [P pol g3], vtheta = a ([dP dpol] - asin m' [P tor']),
 // vphi = a (-[dP tor] - asin m' [P dpol']), with primes on the conjugate slot.
+// VEC with NB = 4: the vector and one scalar (component nd_scl + group)
+// in one work-group: the scalar's product [P s] shares the P slices already
+// in LDS, so a separate scalar launch, which would load them again for a
+// fifth of the products, is not needed.
 template <int S, int NB>
 struct BwdLds {
   static constexpr int NBB = 2 * NB * S;  // data blocks [slot][component][digit]
@@ -684,12 +688,13 @@ template <int S, int NB, bool VEC>
 __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
     SplitDims d, int nd_first, SplitTabs tb, const long long *__restrict__ cb,
     const signed char *__restrict__ dig, const int *__restrict__ dexp, double *__restrict__ out,
-    GridAddr ga, const double *__restrict__ a_r, const double *__restrict__ asin_t) {
+    GridAddr ga, const double *__restrict__ a_r, const double *__restrict__ asin_t, int nd_scl) {
   extern __shared__ i4 smem[];
   using LY = BwdLds<S, NB>;
   constexpr bool BATCH_B = true;  // data slices of a quantity read together
-  constexpr int NQ = VEC ? 5 : NB;
-  constexpr int NO = VEC ? 3 : NB;
+  constexpr bool SCL = VEC && NB == 4;  // vector plus one scalar
+  constexpr int NQ = VEC ? (SCL ? 6 : 5) : NB;
+  constexpr int NO = NB;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform per wave
   // logical id = ((M n_rt + rt) ncg + cgl) nblk + tbk: a run is a whole
@@ -709,7 +714,9 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
   const int tt = wave & 3, sg = wave >> 2;
   const int t0 = (tbk * 4 + tt) * 16;
   const bool active = t0 < d.nth_h && sg < nsig;
-  const int nd0 = nd_first + NB * cgl;
+  const int nd0 = nd_first + (VEC ? 3 : NB) * cgl;
+  // component of block f: the vector's three, then the group's scalar
+  auto comp = [&](int f) { return (SCL && f == 3) ? nd_scl + cgl : nd0 + f; };
   const int ntile = d.nr_p >> 4;
   const int sgc = nsig == 1 ? 0 : 1 - sg;  // conjugate slot
   const int sl_own = sg == 0 ? L + M : L - M;
@@ -741,7 +748,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
       const signed char *base;
       if (blk < LY::NBB) {
         const int sgb = blk / (NB * S), f = (blk / S) % NB, s = blk % S;
-        const int gb = (cbase(sgb, par) + c) * d.ncomp + nd0 + f;
+        const int gb = (cbase(sgb, par) + c) * d.ncomp + comp(f);
         base = dig + (size_t)((gb * S + s) * ntile + rt) * 1024;
       } else if (blk < LY::NBB + LY::NPB) {
         const int t2 = (blk - LY::NBB) / S, s = (blk - LY::NBB) % S;
@@ -768,7 +775,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
       eoff = 0;
       if (sgf < 2 * NB) {
         const int sgb = sgf / NB, f = sgf % NB;
-        eoff = 4u * (unsigned)(((cbase(sgb, par) + c) * d.ncomp + nd0 + f) * d.nr_p + rt * 16 +
+        eoff = 4u * (unsigned)(((cbase(sgb, par) + c) * d.ncomp + comp(f)) * d.nr_p + rt * 16 +
                                (lane & 15));
       }
       edst = exa + 4u * (128 + (wave - 2) * 64);
@@ -829,27 +836,29 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
         const i4 *bo = buf + sg * NB * S * 64 + lane;
         const i4 *bc = buf + sgc * NB * S * 64 + lane;
         // quantity q: A table, data block
-#define AOF(q, s) ((VEC && (q) >= 3) ? adP[s] : aP[s])
+#define AOF(q, s) ((VEC && ((q) == 3 || (q) == 4)) ? adP[s] : aP[s])
 #define BOF(q, t)                                                            \
   (VEC ? ((q) == 0   ? bo[(0 * S + (t)) * 64]                                \
           : (q) == 1 ? bc[(2 * S + (t)) * 64]                                \
           : (q) == 2 ? bc[(1 * S + (t)) * 64]                                \
           : (q) == 3 ? bo[(1 * S + (t)) * 64]                                \
-                     : bo[(2 * S + (t)) * 64])                               \
+          : (q) == 4 ? bo[(2 * S + (t)) * 64]                                \
+                     : bo[(3 * S + (t)) * 64])                               \
        : bo[((q) * S + (t)) * 64])
         // fold of quantity q with the collected T: x = ldexp(T, eA + eB - 20)
         auto fold = [&](int q, const double (&T)[4]) {
           const int eb = VEC ? (q == 0 ? ecol(sg, 0) : q == 1 ? ecol(sgc, 2) : q == 2 ? ecol(sgc, 1)
-                                : q == 3 ? ecol(sg, 1) : ecol(sg, 2))
+                                : q == 3 ? ecol(sg, 1) : q == 4 ? ecol(sg, 2) : ecol(sg, 3))
                              : ecol(sg, q);
-          const int o = VEC ? (q == 0 ? 0 : (q == 1 || q == 3) ? 1 : 2) : q;
+          const int o = VEC ? (q == 0 ? 0 : (q == 1 || q == 3) ? 1 : q == 5 ? 3 : 2) : q;
+          const bool dq = VEC && (q == 3 || q == 4);  // a dP quantity
 #pragma unroll
           for (int r = 0; r < 4; r++) {
-            const int ea = ex[(VEC && q >= 3 ? 64 : 0) + rb + r];
+            const int ea = ex[(dq ? 64 : 0) + rb + r];
             const double x = ldexp(T[r], ea + eb - 20);
             // P terms even in theta about the equator for even l-M, dP terms odd
-            const bool sym = (!VEC || q <= 2) ? (par == 0) : (par == 1);
-            if (!VEC || q == 0 || q == 3) {
+            const bool sym = dq ? (par == 1) : (par == 0);
+            if (!VEC || q == 0 || q == 3 || q == 5) {
               C[o][0][r] += x;
               C[o][1][r] += sym ? x : -x;
             } else if (q == 4) {
@@ -868,7 +877,13 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
           KSHT_SLICED_PRODUCTS(0, 1 % NQ, (NQ > 1 ? 2 : 1), T)
           fold(0, T[0]);
           if (NQ > 1) fold(1 % NQ, T[1]);
-          if (NQ > 2) {
+          if constexpr (SCL) {
+            // the third P quantity together with the scalar's
+            __builtin_amdgcn_sched_barrier(0);
+            KSHT_SLICED_PRODUCTS(2, 5 % NQ, 2, T)
+            fold(2, T[0]);
+            fold(5 % NQ, T[1]);
+          } else if (NQ > 2) {
             __builtin_amdgcn_sched_barrier(0);
             KSHT_SLICED_PRODUCTS(2 % NQ, 2 % NQ, 1, T)
             fold(2 % NQ, T[0]);
@@ -908,8 +923,8 @@ __global__ void __launch_bounds__(kGemmThreads) k_leg_bwd_split(
     const long long os = base + (long long)(d.nth - 1 - t) * ga.s_t;
 #pragma unroll
     for (int o = 0; o < NO; o++) {
-      const double scl = VEC ? (o == 0 ? a2 : a1) : 1.0;
-      double *dst = out + (long long)(nd0 + o) * ga.s_nd;
+      const double scl = VEC ? (o == 0 ? a2 : o < 3 ? a1 : 1.0) : 1.0;
+      double *dst = out + (long long)comp(o) * ga.s_nd;
       if (eq) {
         dst[on] = scl * (0.5 * (C[o][0][r] + C[o][1][r]));
       } else {
@@ -1572,7 +1587,8 @@ static int table_layouts(sht_plan *p, SplitState *st) {
 
 template <int S>
 static constexpr int bwd_lds(int nb) {
-  return nb == 3 ? BwdLds<S, 3>::bytes() : nb == 2 ? BwdLds<S, 2>::bytes() : BwdLds<S, 1>::bytes();
+  return nb == 4 ? BwdLds<S, 4>::bytes() : nb == 3 ? BwdLds<S, 3>::bytes()
+         : nb == 2 ? BwdLds<S, 2>::bytes() : BwdLds<S, 1>::bytes();
 }
 template <int S>
 static constexpr int fwd_lds(int nb) {
@@ -1651,6 +1667,7 @@ static int slice_tables(sht_plan *p, SplitState *st) {
 #define KSHT_LDS(K, BYTES) \
   KSHT_HIP(hipFuncSetAttribute((const void *)(K), hipFuncAttributeMaxDynamicSharedMemorySize, (BYTES)))
   KSHT_LDS((k_leg_bwd_split<S, 3, true>), bwd_lds<S>(3));
+  if constexpr (bwd_lds<S>(4) <= 160 * 1024) KSHT_LDS((k_leg_bwd_split<S, 4, true>), bwd_lds<S>(4));
   KSHT_LDS((k_leg_bwd_split<S, 3, false>), bwd_lds<S>(3));
   KSHT_LDS((k_leg_bwd_split<S, 2, false>), bwd_lds<S>(2));
   KSHT_LDS((k_leg_bwd_split<S, 1, false>), bwd_lds<S>(1));
@@ -1741,13 +1758,24 @@ static SplitDims split_dims(const sht_plan *p, int ncomp, int nblk) {
 // component groups of a call: vectors as (pol, dpol, tor) triples, scalars in
 // threes plus a remainder group of one or two
 struct GroupLaunch {
-  int nb, vec, ncg, nd_first;
+  int nb, vec, ncg, nd_first, nd_scl;  // nd_scl: first fused scalar (nb = 4)
 };
 static int component_groups(int nv, int ns, GroupLaunch g[3]) {
   int n = 0;
-  if (nv) g[n++] = GroupLaunch{3, 1, nv, 0};
-  if (ns / 3) g[n++] = GroupLaunch{3, 0, ns / 3, 3 * nv};
-  if (ns % 3) g[n++] = GroupLaunch{ns % 3, 0, 1, 3 * nv + 3 * (ns / 3)};
+  if (nv) g[n++] = GroupLaunch{3, 1, nv, 0, 0};
+  if (ns / 3) g[n++] = GroupLaunch{3, 0, ns / 3, 3 * nv, 0};
+  if (ns % 3) g[n++] = GroupLaunch{ns % 3, 0, 1, 3 * nv + 3 * (ns / 3), 0};
+  return n;
+}
+// backward groups with the first nf scalars fused into the first nf vector
+// groups (one each), then the remaining vectors and scalars
+static int component_groups_fused(int nv, int ns, int nf, GroupLaunch g[4]) {
+  int n = 0;
+  if (nf) g[n++] = GroupLaunch{4, 1, nf, 0, 3 * nv};
+  if (nv > nf) g[n++] = GroupLaunch{3, 1, nv - nf, 3 * nf, 0};
+  const int r = ns - nf, s0 = 3 * nv + nf;
+  if (r / 3) g[n++] = GroupLaunch{3, 0, r / 3, s0, 0};
+  if (r % 3) g[n++] = GroupLaunch{r % 3, 0, 1, s0 + 3 * (r / 3), 0};
   return n;
 }
 
@@ -1764,8 +1792,10 @@ static int bwd_split_S(sht_plan *p, const Workspace &ws, int nv, int ns, const d
                      p->L, p->nr, p->nr_p, nrad, nv, ncomp, spec, sa, p->t.jst, st->cb, ws.dig,
                      ws.dexp);
   KSHT_HIP(hipGetLastError());
-  GroupLaunch g[3];
-  const int ng = component_groups(nv, ns, g);
+  GroupLaunch g[4];
+  constexpr bool can_fuse = bwd_lds<S>(4) <= 160 * 1024;
+  const int nf = can_fuse && p->var_bwd_fused ? (nv < ns ? nv : ns) : 0;
+  const int ng = component_groups_fused(nv, ns, nf, g);
   for (int ti = 0; ti < ptile_count(p); ti++) {
     SplitTabs tb;
     int M0 = 0, nM = 0;
@@ -1786,8 +1816,13 @@ static int bwd_split_S(sht_plan *p, const Workspace &ws, int nv, int ns, const d
 #define KSHT_BWD(NB, VEC)                                                                      \
   hipLaunchKernelGGL((k_leg_bwd_split<S, NB, VEC>), grid, blk, bwd_lds<S>(NB), sm, d,        \
                      g[i].nd_first, tb, st->cb, ws.dig, ws.dexp, out, ga, p->t.a_r,          \
-                     p->t.asin_t)
-      if (g[i].vec) {
+                     p->t.asin_t, g[i].nd_scl)
+      if (g[i].vec && g[i].nb == 4) {
+        if constexpr (can_fuse) {
+          StageTimer tk(p, 5);
+          KSHT_BWD(4, true);
+        }
+      } else if (g[i].vec) {
         StageTimer tk(p, 5);
         KSHT_BWD(3, true);
       } else if (g[i].nb == 3)

@@ -132,3 +132,34 @@ def test_forward_mfma32_bitexact_vs_mfma16(case, torch_cuda, S, nv, ns):
         assert rel_comp(out[32], ref, nc) <= TOL[S]
     finally:
         plan.close()
+
+
+@pytest.mark.parametrize("S", [4, 5, 6, 7])
+@pytest.mark.parametrize("nv,ns", [(2, 2), (1, 1), (2, 1), (1, 4), (3, 0)])
+def test_backward_fused_scalars_bitexact(case, torch_cuda, S, nv, ns):
+    """A vector launch that also takes one scalar per vector sums the same
+    exact digit products and folds them in the same order as separate
+    scalar launches: both variants agree bit for bit (7 slices have no fused
+    form and take the separate launches either way), and the fused result is
+    held to the oracle bar."""
+    name, cfg, orc = case
+    t = torch_cuda
+    nc = 3 * nv + ns
+    print(f"[gpu] split backward fused {name} S={S} nv={nv} ns={ns}", flush=True)
+    plan = SphTransPlan(cfg, max_ncomp=nc, precision=SHT_PREC_SPLIT_INT8, n_slices=S)
+    try:
+        assert plan.get_variant("split_bwd_fused") == 1
+        sp = synthetic_spectra(cfg, nv, ns, seed=47)
+        d_sp = t.from_numpy(sp).cuda()
+        out = {}
+        for v in (1, 0):
+            plan.set_variant("split_bwd_fused", v)
+            o = t.full((cfg.n_rtm(nc),), np.nan, dtype=t.float64, device="cuda")
+            plan.legendre_backward(nv, ns, d_sp, o)
+            t.cuda.synchronize()
+            out[v] = o.cpu().numpy()
+        assert np.array_equal(out[1], out[0])
+        ref = orc.leg_bwd(nv, ns, sp)
+        assert rel_comp(out[1], ref, nc) <= TOL[S]
+    finally:
+        plan.close()
