@@ -8,8 +8,11 @@
 // to kBranches parallel branches forked from and joined to one capture
 // stream (calls that share a plan stay on one branch, in order), so
 // independent shells may run side by side, and the whole set starts with a
-// single hipGraphLaunch.  Replays reuse the captured buffer
-// pointers; the caller refills the inputs between launches.
+// single hipGraphLaunch.  Plans go to branches largest first, each to the
+// branch with the least estimated work so far, so that the branches end
+// together.  Replays reuse the captured buffer pointers; the caller refills
+// the inputs between launches.
+#include <algorithm>
 #include <vector>
 
 #include "sht_internal.h"
@@ -18,6 +21,19 @@ using namespace ksht;
 
 namespace {
 constexpr int kBranches = 4;
+
+// Work estimate of one call, for balancing the branches only: the Legendre
+// contractions (nr nth (L+1)^2 per component, 5 per vector) and the zonal
+// transform (nr nth nphi per component, weighted as measured: about 200
+// Legendre multiply-adds per FFT point and component).
+double call_cost(const sht_shell_call &c) {
+  const sht_plan *p = c.plan;
+  const double lines = (double)p->nr * p->nth, l2 = (double)(p->L + 1) * (p->L + 1);
+  auto dir = [&](int nv, int ns) {
+    return lines * (l2 * (5.0 * nv + ns) + 200.0 * p->nphi * (3.0 * nv + ns));
+  };
+  return dir(c.nvector_b, c.nscalar_b) + dir(c.nvector_f, c.nscalar_f);
+}
 }
 
 struct sht_batch {
@@ -105,17 +121,40 @@ extern "C" int sht_batch_create(const sht_shell_call *calls, int n, sht_batch **
     if (!hip_ok(hipEventRecord(fork, cap), "fork")) break;
     for (int k = 0; k < kBranches && ok; k++) ok = hip_ok(hipStreamWaitEvent(br[k], fork, 0), "fork");
     if (!ok) break;
-    // calls on one plan share its workspace: they go to the branch of the
-    // plan's first call, in order; distinct plans are dealt round-robin
-    std::vector<int> branch(n);
-    int next = 0;
+    // calls on one plan share its workspace: they go to one branch, in
+    // their order.  Plans in decreasing estimated work, each to the least
+    // loaded branch; calls captured in that plan order
+    std::vector<int> first(n), order(n), branch(n);
+    std::vector<double> pcost(n, 0.0);
     for (int i = 0; i < n; i++) {
-      branch[i] = -1;
-      for (int k = 0; k < i && branch[i] < 0; k++)
-        if (calls[k].plan == calls[i].plan) branch[i] = branch[k];
-      if (branch[i] < 0) branch[i] = next++ % kBranches;
+      first[i] = i;
+      for (int k = 0; k < i; k++)
+        if (calls[k].plan == calls[i].plan) {
+          first[i] = first[k];
+          break;
+        }
+      pcost[first[i]] += call_cost(calls[i]);
     }
-    for (int i = 0; i < n && !rc; i++) {
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      const double ca = pcost[first[a]], cb = pcost[first[b]];
+      return ca != cb ? ca > cb : first[a] < first[b];
+    });
+    double load[kBranches] = {};
+    for (int oi = 0; oi < n; oi++) {
+      const int i = order[oi];
+      if (first[i] == i) {
+        int k0 = 0;
+        for (int k = 1; k < kBranches; k++)
+          if (load[k] < load[k0]) k0 = k;
+        branch[i] = k0;
+        load[k0] += pcost[i];
+      } else {
+        branch[i] = branch[first[i]];
+      }
+    }
+    for (int oi = 0; oi < n && !rc; oi++) {
+      const int i = order[oi];
       const sht_shell_call &c = calls[i];
       c.plan->stream = br[branch[i]];
       if (3 * c.nvector_b + c.nscalar_b)
