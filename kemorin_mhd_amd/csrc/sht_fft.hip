@@ -417,26 +417,25 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_fwd(
 // The backward runs NC = 4 (launch bounds: at least two waves per SIMD, i.e.
 // at most 256 VGPRs).  The forward keeps NC = 8, one work-group per CU, and
 // overlaps instead the copy of the next phi decimation with the passes of
-// the current one inside the work-group; its alias twiddles come from a
-// full N-entry table (composing them from the NP roots, as the backward
-// does, would not fit its register budget).  NP = 1024 (L up to 1023): a
-// complex line spans two waves (T = 128), the passes synchronise the work-
-// group, and the forward runs NC = 4 without the in-kernel copy of the next
-// decimation (its stage would not fit beside the N-entry table).
+// the current one inside the work-group (NC = 4 with two work-groups per CU
+// measured the same).  Both take their twiddles from the NP roots and the
+// R0 roots.  NP = 1024 (L up to 1023): a complex line spans two waves
+// (T = 128), the passes synchronise the work-group, and the forward runs
+// NC = 4 with the same in-kernel copy (about 156 KB of LDS).
 constexpr int kFftNCb = 4;
 template <int NP>
 constexpr int fft_ncf() { return NP <= 512 ? 8 : 4; }
 template <int NP>
-constexpr bool fft_fwd_dma() { return NP <= 512; }
+constexpr bool fft_fwd_dma() { return NP <= 1024; }
 
 __device__ __forceinline__ int fpad(int i) { return i + (i >> 3); }
 
-// LDS layout, in double2 units.  The twiddles (backward: NP roots of order
-// NP, then the R0 roots exp(-2 pi i b / N), b < R0; forward: all N roots
-// exp(-2 pi i k / N); copied from global memory at the start) sit past every
-// other area, so that the passes and the alias sums read no global memory: a
-// global load there would be counted with the copies or stores in flight
-// (vmcnt completes in issue order) and wait for them.
+// LDS layout, in double2 units.  The twiddles (NP roots of order NP, then the
+// R0 roots exp(-2 pi i b / N), b < R0; copied from global memory at the
+// start) sit past every other area, so that the passes and the alias sums
+// read no global memory: a global load there would be counted with the
+// copies or stores in flight (vmcnt completes in issue order) and wait for
+// them.
 //   backward: coefficient block (2 NP rows of NC slots) or staging of
 //             256-order chunks (rows of 2 NC + 1 doubles), then the pass
 //             areas and the output staging (NC + 1 slots per row), reused;
@@ -487,7 +486,7 @@ struct TwoLevelRoots {
 };
 
 // tw: roots of unity in LDS, of order NP at stride ts (the NP-entry table
-// with ts = 1, or the N-entry table with ts = R0).  A butterfly's twiddles
+// with ts = 1).  A butterfly's twiddles
 // w^r, r < R, are powers of its one root w: one table read and R - 2 complex
 // products instead of R - 1 reads, whose strides (k r entries apart across
 // the lanes) made the LDS serve them in two to four passes.
@@ -774,15 +773,17 @@ __global__ void __launch_bounds__(NC * NP / 8, 2) k_fft3_bwd(int N, int R0, unsi
 // Forward accumulation of decimation j1: e^{-2 pi i k j1 / N} for k = j + T q
 // as e^{-2 pi i j j1 / N} times the q-th power of e^{-2 pi i T j1 / N} (three
 // table reads per thread instead of sixteen), the alias k + N - NP one factor
-// e^{-2 pi i (N - NP j1) / N} further.  twl: exp(-2 pi i k / N), k < N.
+// e^{-2 pi i (N - NP j1) / N} further, each from the NP roots and the R0
+// roots (tw_n).
 template <int NP>
 __device__ __forceinline__ void f3_fwd_alias(const double2 (&w)[8], double2 (&zlo)[8],
-                                             double2 (&zhi)[8], const double2 *twl, int j,
+                                             double2 (&zhi)[8], const double2 *twp,
+                                             const double2 *twr, int R0, unsigned r0m, int j,
                                              int j1, int N, int L) {
   constexpr int T = NP / 8;
-  double2 t = twl[j * j1];
-  const double2 st = twl[T * j1];
-  const double2 hf = j1 == 0 ? make_double2(1.0, 0.0) : twl[N - NP * j1];
+  double2 t = tw_n(twp, twr, j * j1, R0, r0m);
+  const double2 st = tw_n(twp, twr, T * j1, R0, r0m);
+  const double2 hf = j1 == 0 ? make_double2(1.0, 0.0) : tw_n(twp, twr, N - NP * j1, R0, r0m);
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     const int k = j + T * q;
@@ -794,11 +795,9 @@ __device__ __forceinline__ void f3_fwd_alias(const double2 (&w)[8], double2 (&zl
 }
 
 template <int NP, int NC>
-__global__ void __launch_bounds__(NC * NP / 8) k_fft3_fwd(int N, int R0, int L,
-                                                          long long nline,
-                                                          const double *__restrict__ v,
-                                                          double *__restrict__ X, FourierAddr xa,
-                                                          const double2 *__restrict__ twn) {
+__global__ void __launch_bounds__(NC * NP / 8, NC * NP / 8 <= 256 ? 2 : 1)
+    k_fft3_fwd(int N, int R0, unsigned r0m, int L, long long nline, const double *__restrict__ v,
+               double *__restrict__ X, FourierAddr xa, const double2 *__restrict__ twn) {
   extern __shared__ double2 lds[];
   constexpr int T = NP / 8, NT = NC * T, W = 2 * NC, SS = W + 1;
   const int tid = threadIdx.x, cl = tid / T, j = tid % T;
@@ -810,15 +809,19 @@ __global__ void __launch_bounds__(NC * NP / 8) k_fft3_fwd(int N, int R0, int L,
 #pragma unroll
   for (int q = 0; q < 8; q++) zlo[q] = zhi[q] = make_double2(0.0, 0.0);
   double2 *ln = lds + cl * (NP + NP / 8);
-  double2 *twl = lds + fft3_tw_off_fwd<NP, NC>();
-  for (int i = tid; i < N; i += NT) twl[i] = twn[i];  // read after the first barrier below
+  // NP roots of order NP, then the R0 roots exp(-2 pi i b / N), b < R0, as
+  // in the backward; read after the first barrier below
+  double2 *twp = lds + fft3_tw_off_fwd<NP, NC>(), *twr = twp + NP;
+  for (int i = tid; i < NP + R0; i += NT) twp[i] = twn[i < NP ? R0 * i : i - NP];
   // DMA form (lines contiguous in pairs, aligned base, per-component byte
   // offsets within 32 bits): the rows of the next phi decimation j1 + 1 are
   // copied global -> LDS while the FFT of j1 runs, into a stage past the
   // pass areas; row r's slot u holds lines 2 (u ^ (r & (NC - 1))) and +1
   // (XOR swizzle: consecutive threads of a complex line read distinct slots).
+  // the copies address each wave instruction's 64 / NC rows from a base of
+  // their own, so a lane's byte offset stays below R0 (64 / NC) nline 8
   const bool dma = fft_fwd_dma<NP>() && vec2 && (((unsigned long long)v & 15) == 0) &&
-                   nnod * 8 < (1ll << 32);
+                   (long long)R0 * (64 / NC) * nline * 8 < (1ll << 32);
   if (dma) {
     double2 *st2 = lds + NC * (NP + NP / 8);
     const unsigned st2_byte =
@@ -830,8 +833,9 @@ __global__ void __launch_bounds__(NC * NP / 8) k_fft3_fwd(int N, int R0, int L,
       for (int i0 = tid & ~63; i0 < NC * NP; i0 += NT) {
         const int i = i0 + (tid & 63), r = i / NC, u = i % NC, c = u ^ (r & (NC - 1));
         const long long cc = line0 + 2 * c + 1 < nline ? 2 * c : 0;  // past the end: line0
-        const unsigned off = (unsigned)(((long long)R0 * r * nline + cc) * 8);
-        fft_dma16(base, off, st2_byte + 16u * (unsigned)i0);
+        const int r0w = i0 / NC;  // first row of this wave instruction (uniform)
+        const unsigned off = (unsigned)(((long long)R0 * (r - r0w) * nline + cc) * 8);
+        fft_dma16(base + (long long)R0 * r0w * nline, off, st2_byte + 16u * (unsigned)i0);
       }
     };
     issue(0);
@@ -846,8 +850,8 @@ __global__ void __launch_bounds__(NC * NP / 8) k_fft3_fwd(int N, int R0, int L,
       }
       __syncthreads();
       if (j1 + 1 < R0) issue(j1 + 1);
-      f3_passes<NP, 1>(w, ln, j, twl, R0, -1.0);
-      f3_fwd_alias<NP>(w, zlo, zhi, twl, j, j1, N, L);
+      f3_passes<NP, 1>(w, ln, j, twp, 1, -1.0);
+      f3_fwd_alias<NP>(w, zlo, zhi, twp, twr, R0, r0m, j, j1, N, L);
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   } else
@@ -884,8 +888,8 @@ __global__ void __launch_bounds__(NC * NP / 8) k_fft3_fwd(int N, int R0, int L,
 #pragma unroll
     for (int q = 0; q < 8; q++) w[q] = lds[(j + T * q) * (NC + 1) + cl];
     __syncthreads();
-    f3_passes<NP, 1>(w, ln, j, twl, R0, -1.0);
-    f3_fwd_alias<NP>(w, zlo, zhi, twl, j, j1, N, L);
+    f3_passes<NP, 1>(w, ln, j, twp, 1, -1.0);
+    f3_fwd_alias<NP>(w, zlo, zhi, twp, twr, R0, r0m, j, j1, N, L);
   }
   // Z_{N-m} for m = j + T q sits at index NP - m of the lane (T - j) % T of
   // this complex line, in register 7 - q (or (8 - q) % 8 when j = 0)
@@ -1569,7 +1573,7 @@ int fft_plan_init(sht_plan *p) {
     }
 #undef KSHT_TW_OFF
     fp.lds3b = (int)sizeof(double2) * (tw_b + np + fp.r0);
-    fp.lds3 = (int)sizeof(double2) * (tw_f + n);
+    fp.lds3 = (int)sizeof(double2) * (tw_f + np + fp.r0);
     fp.r0m = fp.r0 == 1 ? 0u : (unsigned)(((1ull << 32) + (unsigned)fp.r0 - 1) / (unsigned)fp.r0);
     int dev = 0, lds_max = 0;
     KSHT_HIP(hipGetDevice(&dev));
@@ -1670,7 +1674,7 @@ int launch_fft_fwd(sht_plan *p, int ncomp, const double *v, double *X, FourierAd
   const size_t lds = sizeof(double2) * (size_t)(B / 2) * p->fft.line_elems;
   if (p->fft.fast && p->var_fft_form) {
 #define KSHT_F(NP) KSHT_FFT3_CASE(k_fft3_fwd, NP, fft_ncf<NP>(), p->fft.lds3, p->nphi, p->fft.r0, \
-                                  p->L, nline, v, X, xa, p->t.twn)
+                                  p->fft.r0m, p->L, nline, v, X, xa, p->t.twn)
     switch (p->fft.np) {
       KSHT_F(32)
       KSHT_F(64)
