@@ -6,7 +6,8 @@ import numpy as np
 import pytest
 
 import oracle as O
-from kemorin_mhd_amd import SHT_PREC_SPLIT_INT8, ShellConfig, SphTransPlan, synthetic_spectra
+from kemorin_mhd_amd import (SHT_PREC_SPLIT_INT8, ShellConfig, ShtError, SphTransPlan,
+                             synthetic_spectra)
 
 pytestmark = pytest.mark.gpu
 
@@ -148,6 +149,9 @@ def test_backward_fused_scalars_bitexact(case, torch_cuda, S, nv, ns):
     print(f"[gpu] split backward fused {name} S={S} nv={nv} ns={ns}", flush=True)
     plan = SphTransPlan(cfg, max_ncomp=nc, precision=SHT_PREC_SPLIT_INT8, n_slices=S)
     try:
+        assert plan.get_variant("split_bwd_fused") == 1
+        with pytest.raises(ShtError):
+            plan.set_variant("split_bwd_fused", 2)
         assert plan.get_variant("split_bwd_fused") == 1
         sp = synthetic_spectra(cfg, nv, ns, seed=47)
         d_sp = t.from_numpy(sp).cuda()
