@@ -422,7 +422,15 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_fwd(
 // R0 roots.  NP = 1024 (L up to 1023): a complex line spans two waves
 // (T = 128), the passes synchronise the work-group, and the forward runs
 // NC = 4 with the same in-kernel copy (about 156 KB of LDS).
-constexpr int kFftNCb = 4;
+// backward complex lines per work-group: 4, and 2 at NP = 1024, where the
+// whole-block staging of 2 NP rows then leaves room for two work-groups per
+// CU once the twiddles are two-level factors (NP/64 + 64 entries)
+template <int NP>
+constexpr int fft_ncb() { return NP >= 1024 ? 2 : 4; }
+template <int NP>
+constexpr bool fft_bwd_2l() { return NP >= 1024; }
+template <int NP>
+constexpr int fft_bwd_tw_entries() { return fft_bwd_2l<NP>() ? NP / 64 + 64 : NP; }
 template <int NP>
 constexpr int fft_ncf() { return NP <= 512 ? 8 : 4; }
 template <int NP>
@@ -590,7 +598,9 @@ __device__ __forceinline__ constexpr int f3_pos(int q) {
 // exp(-2 pi i k / N), k < N = R0 NP, as exp(-2 pi i a / NP) exp(-2 pi i b / N)
 // with k = R0 a + b; a = k / R0 by a multiply-high with r0m = ceil(2^32 / R0)
 // (exact for k < 2^32 / R0; r0m is unused for R0 = 1)
-__device__ __forceinline__ double2 tw_n(const double2 *twp, const double2 *twr, int k, int R0,
+// (twp: the NP roots as a table or as two-level factors)
+template <class TP>
+__device__ __forceinline__ double2 tw_n(const TP &twp, const double2 *twr, int k, int R0,
                                         unsigned r0m) {
   const int a = R0 == 1 ? k : (int)__umulhi((unsigned)k, r0m);
   return cmul(twp[a], twr[k - a * R0]);
@@ -628,9 +638,18 @@ __global__ void __launch_bounds__(NC * NP / 8, 2) k_fft3_bwd(int N, int R0, unsi
   const int nd = blockIdx.y;
   const long long line0 = (long long)xcd_block(blockIdx.x, gridDim.x) * W;
   double *stage = reinterpret_cast<double *>(lds);
-  double2 *twp = lds + fft3_tw_off_bwd<NP, NC>(), *twr = twp + NP;
-  // read after the staging barrier below
-  for (int i = tid; i < NP + R0; i += NT) twp[i] = twn[i < NP ? R0 * i : i - NP];
+  constexpr bool TL = fft_bwd_2l<NP>();
+  constexpr int NTW = fft_bwd_tw_entries<NP>();
+  double2 *twp = lds + fft3_tw_off_bwd<NP, NC>(), *twr = twp + NTW;
+  // the NP roots exp(-2 pi i a / NP) (as hi[a / 64] lo[a % 64] when TL),
+  // then the R0 roots; read after the staging barrier below
+  for (int i = tid; i < NTW + R0; i += NT) {
+    const int src = i >= NTW ? i - NTW
+                    : !TL    ? R0 * i
+                    : i < NP / 64 ? R0 * 64 * i : R0 * (i - NP / 64);
+    twp[i] = twn[src];
+  }
+  const TwoLevelRoots tl{twp, twp + NP / 64};
   double2 zlo[8], zhi[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) zlo[q] = zhi[q] = make_double2(0.0, 0.0);
@@ -733,9 +752,15 @@ __global__ void __launch_bounds__(NC * NP / 8, 2) k_fft3_bwd(int N, int R0, unsi
     // e^{+2 pi i m j1 / N} for m = j + T q as e^{+2 pi i j j1 / N} times the
     // q-th power of e^{+2 pi i T j1 / N}, and the alias m + N - NP one factor
     // e^{-2 pi i NP j1 / N} further
-    double2 t1 = tw_n(twp, twr, j * j1, R0, r0m);
-    double2 st = tw_n(twp, twr, T * j1, R0, r0m);
-    double2 hf = j1 == 0 ? make_double2(1.0, 0.0) : tw_n(twp, twr, N - NP * j1, R0, r0m);
+    auto root_n = [&](int k) {  // exp(-2 pi i k / N)
+      if constexpr (TL)
+        return tw_n(tl, twr, k, R0, r0m);
+      else
+        return tw_n(static_cast<const double2 *>(twp), twr, k, R0, r0m);
+    };
+    double2 t1 = root_n(j * j1);
+    double2 st = root_n(T * j1);
+    double2 hf = j1 == 0 ? make_double2(1.0, 0.0) : root_n(N - NP * j1);
     t1.y = -t1.y;  // e^{+...}
     st.y = -st.y;
     hf.y = -hf.y;
@@ -744,7 +769,10 @@ __global__ void __launch_bounds__(NC * NP / 8, 2) k_fft3_bwd(int N, int R0, unsi
       w[q] = cfma(zhi[q], cmul(t1, hf), cmul(zlo[q], t1));
       if (q < 7) t1 = cmul(t1, st);
     }
-    f3_passes<NP, 1>(w, ln, j, twp, 1, 1.0);
+    if constexpr (TL)
+      f3_passes_2l<NP, 1>(w, ln, j, tl, 1.0);
+    else
+      f3_passes<NP, 1>(w, ln, j, twp, 1, 1.0);
     __syncthreads();  // every wave done with its pass area (under the staging)
     // rows phi = R0 (j + T q) + j1, staged as [j2][NC complex lines] (stride NC + 1)
 #pragma unroll
@@ -1559,7 +1587,7 @@ int fft_plan_init(sht_plan *p) {
     int tw_b = 0, tw_f = 0;
 #define KSHT_TW_OFF(NP)                                                                      \
   case NP:                                                                                   \
-    tw_b = fft3_tw_off_bwd<NP, kFftNCb>();                                                   \
+    tw_b = fft3_tw_off_bwd<NP, fft_ncb<NP>()>() + fft_bwd_tw_entries<NP>();                   \
     tw_f = fft3_tw_off_fwd<NP, fft_ncf<NP>()>();                                             \
     break;
     switch (np) {
@@ -1572,7 +1600,7 @@ int fft_plan_init(sht_plan *p) {
       default: break;
     }
 #undef KSHT_TW_OFF
-    fp.lds3b = (int)sizeof(double2) * (tw_b + np + fp.r0);
+    fp.lds3b = (int)sizeof(double2) * (tw_b + fp.r0);
     fp.lds3 = (int)sizeof(double2) * (tw_f + np + fp.r0);
     fp.r0m = fp.r0 == 1 ? 0u : (unsigned)(((1ull << 32) + (unsigned)fp.r0 - 1) / (unsigned)fp.r0);
     int dev = 0, lds_max = 0;
@@ -1592,7 +1620,7 @@ int fft_plan_init(sht_plan *p) {
     p->device_bytes += (long long)(sizeof(double2) * n);
 #define KSHT_FFT3_ATTR(NP)                                                                   \
   case NP:                                                                                   \
-    KSHT_HIP(hipFuncSetAttribute((const void *)k_fft3_bwd<NP, kFftNCb>,                      \
+    KSHT_HIP(hipFuncSetAttribute((const void *)k_fft3_bwd<NP, fft_ncb<NP>()>,                \
                                  hipFuncAttributeMaxDynamicSharedMemorySize, fp.lds3b));     \
     KSHT_HIP(hipFuncSetAttribute((const void *)k_fft3_fwd<NP, fft_ncf<NP>()>,                \
                                  hipFuncAttributeMaxDynamicSharedMemorySize, fp.lds3));      \
@@ -1647,7 +1675,7 @@ int launch_fft_bwd(sht_plan *p, int ncomp, const double *X, FourierAddr xa, doub
     }                                                                                      \
   } while (0)
   if (p->fft.fast && p->var_fft_form) {
-#define KSHT_B(NP) KSHT_FFT3_CASE(k_fft3_bwd, NP, kFftNCb, p->fft.lds3b, p->nphi, p->fft.r0,   \
+#define KSHT_B(NP) KSHT_FFT3_CASE(k_fft3_bwd, NP, fft_ncb<NP>(), p->fft.lds3b, p->nphi, p->fft.r0, \
                                   p->fft.r0m, p->L, nline, X, xa, v, p->t.twn)
     switch (p->fft.np) {
       KSHT_B(32)
