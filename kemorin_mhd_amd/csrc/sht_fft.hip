@@ -415,7 +415,7 @@ __global__ void __launch_bounds__(kFftThreads) k_fft_fwd(
 // neighbouring logical blocks, which xcd_block keeps on one XCD (one L2)
 // at about the same time.
 // The backward runs NC = 4 (launch bounds: at least two waves per SIMD, i.e.
-// at most 256 VGPRs).  The forward keeps NC = 8, one work-group per CU, and
+// at most 256 VGPRs), NC = 2 at NP = 1024 (fft_ncb).  The forward keeps NC = 8, one work-group per CU, and
 // overlaps instead the copy of the next phi decimation with the passes of
 // the current one inside the work-group (NC = 4 with two work-groups per CU
 // measured the same).  Both take their twiddles from the NP roots and the
