@@ -40,7 +40,8 @@ enum {
   SHT_ERR_UNSUPPORTED = 2, /* e.g. FFT length beyond the LDS capacity   */
   SHT_ERR_OOM = 3,         /* device allocation failed               */
   SHT_ERR_HIP = 4,         /* HIP runtime error                      */
-  SHT_ERR_STATE = 5        /* plan not ready / wrong precision mode  */
+  SHT_ERR_STATE = 5,       /* plan not ready / wrong precision mode  */
+  SHT_ERR_NONFINITE = 6    /* non-finite output (SHT_FLAG_CHECK_FINITE) */
 };
 
 enum { SHT_PREC_FP64 = 0, SHT_PREC_SPLIT_INT8 = 1 };
@@ -68,6 +69,14 @@ typedef struct {
  * slices), at the cost of the recurrence on every call.  Results are bit for
  * bit those of stored tables. */
 #define SHT_FLAG_P_ON_THE_FLY 1
+/* SHT_FLAG_CHECK_FINITE: after each sht_backward / sht_forward (and their
+ * _host and _noalloc forms) count the NaN and infinite values of the output
+ * on the device and return SHT_ERR_NONFINITE, the count in the message, if
+ * there are any (the output is still written).  Each call then waits for
+ * the plan's stream.  A call being captured into a graph skips the check.
+ * For debugging runs; the reference has no such check (its routines return
+ * nothing). */
+#define SHT_FLAG_CHECK_FINITE 2
 
 /* Replaces initialize_sph_trans (SS/init_sph_trans.f90:46-74) and the
  * Legendre/FFT set-up it runs: Gauss points with the reference's Newton

@@ -95,10 +95,15 @@ static int backward_ws(sht_plan *p, const Workspace &ws, int nv, int ns, const d
   return launch_fft_bwd(p, ncomp, ws.X, x_fourier_addr(p), v_rtp);
 }
 
+static int check_finite(sht_plan *p, const double *out, long long n, const char *where);
+
 extern "C" int sht_backward(sht_plan *p, int nv, int ns, const double *WS_rj, double *v_rtp) {
   int rc = check_call(p, nv, ns, WS_rj, v_rtp, "sht_backward");
   if (rc) return rc;
-  return backward_ws(p, plan_workspace(p), nv, ns, WS_rj, v_rtp);
+  rc = backward_ws(p, plan_workspace(p), nv, ns, WS_rj, v_rtp);
+  if (rc) return rc;
+  return check_finite(p, v_rtp, (long long)(3 * nv + ns) * p->nr * p->nth * p->nphi,
+                      "sht_backward");
 }
 
 extern "C" int sht_forward(sht_plan *p, int nv, int ns, const double *v_rtp, double *WR_rj) {
@@ -111,7 +116,9 @@ extern "C" int sht_forward(sht_plan *p, int nv, int ns, const double *v_rtp, dou
     rc = launch_fft_fwd(p, ncomp, v_rtp, ws.X, x_fourier_addr(p));
   }
   if (rc) return rc;
-  return leg_fwd(p, ws, nv, ns, ws.X, x_grid_addr(p), WR_rj, spec_addr(p, ncomp));
+  rc = leg_fwd(p, ws, nv, ns, ws.X, x_grid_addr(p), WR_rj, spec_addr(p, ncomp));
+  if (rc) return rc;
+  return check_finite(p, WR_rj, (long long)ncomp * p->jmax * p->nr, "sht_forward");
 }
 
 // ---------------------------------------------------------------------------
@@ -153,6 +160,41 @@ int scratch_attach(int device) {
 
 static long long align256(long long n) { return (n + 255) / 256 * 256; }
 
+// Number of non-finite values (exponent field all ones: NaN or +-Inf) of
+// x[0 .. n), added to *cnt.
+__global__ void __launch_bounds__(256) k_count_nonfinite(const double *__restrict__ x, long long n,
+                                                         unsigned long long *cnt) {
+  unsigned c = 0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256)
+    c += ((__double_as_longlong(x[i]) >> 52) & 0x7ff) == 0x7ff ? 1u : 0u;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(cnt, (unsigned long long)c);
+}
+
+// SHT_FLAG_CHECK_FINITE: count the output's non-finite values after the call
+// (the plan's stream is waited for); skipped while the stream is captured
+static int check_finite(sht_plan *p, const double *out, long long n, const char *where) {
+  if (!p->nonfinite || n <= 0) return SHT_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  const hipStream_t st = api_stream(p->stream);
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+    return SHT_OK;
+  KSHT_HIP(hipMemsetAsync(p->nonfinite, 0, sizeof(unsigned long long), st));
+  const long long nb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_count_nonfinite, dim3((unsigned)(nb < 4096 ? nb : 4096)), dim3(256), 0,
+                     p->stream, out, n, p->nonfinite);
+  KSHT_HIP(hipGetLastError());
+  unsigned long long h = 0;
+  KSHT_HIP(hipMemcpyAsync(&h, p->nonfinite, sizeof(h), hipMemcpyDeviceToHost, st));
+  KSHT_HIP(hipStreamSynchronize(st));
+  if (h) {
+    set_error("%s: %llu non-finite values in the output (SHT_FLAG_CHECK_FINITE)", where, h);
+    return SHT_ERR_NONFINITE;
+  }
+  return SHT_OK;
+}
+
 extern "C" int sht_backward_noalloc(sht_plan *p, int nv, int ns, const double *WS_rj,
                                     double *v_rtp) {
   int rc = check_call(p, nv, ns, WS_rj, v_rtp, "sht_backward_noalloc");
@@ -190,7 +232,10 @@ extern "C" int sht_backward_noalloc(sht_plan *p, int nv, int ns, const double *W
   unsigned char *b = (unsigned char *)base;
   const Workspace ws{(double *)b, db ? (signed char *)(b + xb) : nullptr,
                      eb ? (int *)(b + xb + db) : nullptr};
-  return backward_ws(p, ws, nv, ns, WS_rj, v_rtp);
+  rc = backward_ws(p, ws, nv, ns, WS_rj, v_rtp);
+  if (rc) return rc;
+  return check_finite(p, v_rtp, (long long)ncomp * p->nr * p->nth * p->nphi,
+                      "sht_backward_noalloc");
 }
 
 static int host_call(sht_plan *p, int nv, int ns, const double *in, double *out, bool bwd) {

@@ -155,6 +155,8 @@ struct sht_plan {
   // split backward: 1 = each vector group's launch also takes one scalar
   // (sharing its P slices), 0 = scalars in launches of their own
   int var_bwd_fused = 1;
+  // SHT_FLAG_CHECK_FINITE: device counter of non-finite outputs (else null)
+  unsigned long long *nonfinite = nullptr;
   // FFT form: 1 = split form n = r0 * 2^k where the plan has it, 0 = the
   // mixed-radix Stockham form for every length, 2 = the pruned real DFT as an
   // FP64 GEMM (set-up on first selection: sht_dft.hip)

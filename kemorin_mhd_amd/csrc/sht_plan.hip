@@ -358,7 +358,7 @@ static void free_plan(sht_plan *p) {
   void *ptrs[] = {p->t.offb, p->t.offf, p->t.ipad, p->t.jst, p->t.Pb, p->t.dPb,
                   p->t.Pf, p->t.dPf, p->t.radius, p->t.a_r, p->t.weight, p->t.asin_t,
                   p->t.cos_t, p->t.sin_t, p->t.tw, p->t.twn, p->t.ztab, p->t.slot_off_X, p->t.slot_off_rtm,
-                  p->t.pole, p->X, p->offb_r, p->offf_r, p->Pt, p->dPt};
+                  p->t.pole, p->X, p->offb_r, p->offf_r, p->Pt, p->dPt, p->nonfinite};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   split_free(p);
@@ -399,9 +399,9 @@ extern "C" int sht_plan_create(const sht_desc *d, sht_plan **out) {
   if (d->precision == SHT_PREC_SPLIT_INT8 && d->n_slices != 0 &&
       (d->n_slices < 4 || d->n_slices > 7))
     BAD("n_slices = %d: need 0 (default) or 4..7 in split mode", d->n_slices);
-  if (d->flags & ~SHT_FLAG_P_ON_THE_FLY)
-    BAD("flags = %d: unknown bits (SHT_FLAG_P_ON_THE_FLY = %d is the only flag)", d->flags,
-        SHT_FLAG_P_ON_THE_FLY);
+  if (d->flags & ~(SHT_FLAG_P_ON_THE_FLY | SHT_FLAG_CHECK_FINITE))
+    BAD("flags = %d: unknown bits (SHT_FLAG_P_ON_THE_FLY = %d, SHT_FLAG_CHECK_FINITE = %d)",
+        d->flags, SHT_FLAG_P_ON_THE_FLY, SHT_FLAG_CHECK_FINITE);
 #undef BAD
   for (int k = 0; k < nr; k++)
     if (!(d->radius[k] > 0.0)) {
@@ -482,6 +482,14 @@ extern "C" int sht_plan_create(const sht_desc *d, sht_plan **out) {
   }
   for (int s = 0; s < p->nslot; s++) so_rtm[s] = (long long)s * nth * nr;  // x ncomp
   CHECK(upload(&p->t.jst, p->lstack.data(), p->nslot, &p->device_bytes));
+  if (d->flags & SHT_FLAG_CHECK_FINITE) {
+    if (hipMalloc(&p->nonfinite, sizeof(unsigned long long)) != hipSuccess) {
+      p->nonfinite = nullptr;
+      set_error("sht_plan_create: device allocation failed (finite-check counter)");
+      CHECK(SHT_ERR_OOM);
+    }
+    p->device_bytes += (long long)sizeof(unsigned long long);
+  }
   CHECK(upload(&p->t.radius, p->radius.data(), nr, &p->device_bytes));
   CHECK(upload(&p->t.a_r, a_r.data(), nr, &p->device_bytes));
   CHECK(upload(&p->t.weight, p->weight.data(), nth, &p->device_bytes));

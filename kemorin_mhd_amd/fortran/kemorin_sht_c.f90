@@ -14,8 +14,12 @@
       integer(c_int), parameter :: SHT_OK = 0
       integer(c_int), parameter :: SHT_PREC_FP64 = 0
       integer(c_int), parameter :: SHT_PREC_SPLIT_INT8 = 1
-!     descriptor flags: P_l^m generated per call, one tile of orders at a time
+!     descriptor flags: P_l^m generated per call, one tile of orders at a time;
+!     count the outputs' non-finite values after each whole transform
       integer(c_int), parameter :: SHT_FLAG_P_ON_THE_FLY = 1
+      integer(c_int), parameter :: SHT_FLAG_CHECK_FINITE = 2
+!     status of a whole transform whose output holds NaN or Inf (with the flag)
+      integer(c_int), parameter :: SHT_ERR_NONFINITE = 6
 !
 !>      Plan descriptor (sht_desc)
       type, bind(C) :: sht_desc

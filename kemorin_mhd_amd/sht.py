@@ -147,6 +147,8 @@ def _check(rc: int, where: str):
 
 HIP_STREAM_LEGACY = 1  # hipStreamLegacy: the legacy default (null) stream
 SHT_FLAG_P_ON_THE_FLY = 1  # P tables generated per call, one tile of orders at a time
+SHT_FLAG_CHECK_FINITE = 2  # whole transforms count their outputs' NaN / Inf values
+SHT_ERR_NONFINITE = 6      # status of a checked transform whose output is not finite
 
 
 def _torch_stream(*bufs):
@@ -219,12 +221,15 @@ class SphTransPlan:
     SS/t_legendre_trans_select.f90 and SS/t_sph_FFT_selector.F90."""
 
     def __init__(self, cfg: ShellConfig, max_ncomp: int = 9, precision: int = SHT_PREC_FP64,
-                 n_slices: int = 0, radius=None, p_on_the_fly: bool = False):
+                 n_slices: int = 0, radius=None, p_on_the_fly: bool = False,
+                 check_finite: bool = False):
         self.cfg = cfg
         r = np.ascontiguousarray(cfg.radius() if radius is None else radius, dtype=np.float64)
         self._radius = r
+        flags = ((SHT_FLAG_P_ON_THE_FLY if p_on_the_fly else 0) |
+                 (SHT_FLAG_CHECK_FINITE if check_finite else 0))
         d = _Desc(cfg.L, cfg.nr, cfg.nth, cfg.nphi, r.ctypes.data_as(_dp), max_ncomp, precision,
-                  n_slices, SHT_FLAG_P_ON_THE_FLY if p_on_the_fly else 0)
+                  n_slices, flags)
         h = _vp()
         _check(lib().sht_plan_create(ctypes.byref(d), ctypes.byref(h)), "sht_plan_create")
         self._h = h
