@@ -1,22 +1,25 @@
 """Config 4 at full size (L=1023, Nr=257, Ntheta=1536, Nphi=3072) against the
-oracle, split engine, P generated per call, with the benchmark's buffer
+oracle, both engines, P generated per call, with the benchmark's buffer
 arrangement (the backward writes its 8 grid components into the first 8 of
 the forward's 9).  Every radius is transformed on the device, so the offsets
 of the full buffers (9.7 GB per grid component) are exercised; radii are
-independent, so the oracle takes two of them (the boundaries).  Bar: split
-1e-11 per component, as on config 4's grid at 2 radii.  The case needs about
-235 GB of device memory and is skipped where less is free."""
+independent, so the oracle takes two of them (the boundaries).  Bars: FP64
+1e-12 and split 1e-11 per component, as on config 4's grid at 2 radii.  The
+cases need about 185 GB (FP64) and 235 GB (split) of device memory and are
+skipped where less is free."""
 import numpy as np
 import pytest
 
 import oracle as O
-from kemorin_mhd_amd import SHT_PREC_SPLIT_INT8, ShellConfig, ShtError, SphTransPlan, rlm_l_m
+from kemorin_mhd_amd import (SHT_PREC_FP64, SHT_PREC_SPLIT_INT8, ShellConfig, ShtError,
+                             SphTransPlan, rlm_l_m)
 
 pytestmark = pytest.mark.gpu
 
 CFG4 = ShellConfig(1023, 257, 1536, 3072)
 KS = np.array([0, 256])
-NEED = 240e9  # split plan with per-call tables + spectra in/out + 9 grid components
+# plan with per-call tables + spectra in / out + 9 grid components
+NEED = {SHT_PREC_FP64: 190e9, SHT_PREC_SPLIT_INT8: 240e9}
 
 
 def _rel(a, b, ncomp, comp_fastest):
@@ -25,18 +28,21 @@ def _rel(a, b, ncomp, comp_fastest):
     return max(np.linalg.norm(a[:, c] - b[:, c]) / np.linalg.norm(b[:, c]) for c in range(ncomp))
 
 
-def test_config4_full_size_split_vs_oracle(torch_cuda):
+@pytest.mark.parametrize("prec,tol", [(SHT_PREC_FP64, 1e-12), (SHT_PREC_SPLIT_INT8, 1e-11)],
+                         ids=["fp64", "split"])
+def test_config4_full_size_vs_oracle(torch_cuda, prec, tol):
     t = torch_cuda
     cfg = CFG4
     t.cuda.empty_cache()
     free, _ = t.cuda.mem_get_info()
-    if free < NEED:
-        pytest.skip(f"config 4 at full size needs about {NEED / 1e9:.0f} GB free, {free / 1e9:.0f} GB is")
-    print("[gpu] config 4 full size, split engine, P per call", flush=True)
+    if free < NEED[prec]:
+        pytest.skip(f"config 4 at full size needs about {NEED[prec] / 1e9:.0f} GB free, "
+                    f"{free / 1e9:.0f} GB is")
+    print(f"[gpu] config 4 full size, precision {prec}, P per call", flush=True)
     plan = None
     try:
         try:
-            plan = SphTransPlan(cfg, max_ncomp=9, precision=SHT_PREC_SPLIT_INT8, p_on_the_fly=True)
+            plan = SphTransPlan(cfg, max_ncomp=9, precision=prec, p_on_the_fly=True)
             gen = t.Generator(device="cuda")
             gen.manual_seed(4401)
             l, _ = rlm_l_m(cfg.L)
@@ -70,4 +76,4 @@ def test_config4_full_size_split_vs_oracle(torch_cuda):
     eb = _rel(grid_s[:8].ravel(), ref_v, 8, comp_fastest=False)
     ef = _rel(out_s, ref_s, 9, comp_fastest=True)
     print(f"    backward {eb:.2e} forward {ef:.2e}", flush=True)
-    assert eb <= 1e-11 and ef <= 1e-11
+    assert eb <= tol and ef <= tol
