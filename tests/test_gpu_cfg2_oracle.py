@@ -5,7 +5,8 @@ L=511, Nr=257, Ntheta=768, Nphi=1536: the plan transforms all 257 radii
 the oracle, in which radii are independent, takes five of them: both
 boundaries, their neighbours and the middle.  Spectra are (l+1)^-1.5 with a
 radial profile 1/2 + sin(pi (r-ri)/(ro-ri)), non-zero at the boundaries so
-that the 1/r and r^2 factors there are exercised.
+that the 1/r and r^2 factors there are exercised; the steep (l+1)^-3
+spectra of config 2 as a second case.
 
 Bars: FP64 engine within relative L2 1e-12 per component of the oracle,
 split engine (6 INT8 slices) within 1e-11, for the whole backward (2 vector
@@ -48,18 +49,20 @@ def _rel(a, b, ncomp, comp_fastest):
             for c in range(ncomp)]
 
 
-@pytest.fixture(scope="module")
-def ref(torch_cuda):
-    """Inputs on the device and the oracle's transforms of them on KS."""
+@pytest.fixture(scope="module", params=[1.5, 3.0], ids=["slope1.5", "slope3"])
+def ref(torch_cuda, request):
+    """Inputs on the device and the oracle's transforms of them on KS, for
+    the standard (l+1)^-1.5 and the steep (l+1)^-3 spectra."""
     t = torch_cuda
     cfg = CFG2
-    _say("[gpu] config 2 grid: inputs and oracle on radii", list(KS))
+    slope = request.param
+    _say("[gpu] config 2 grid: inputs and oracle on radii", list(KS), "slope", slope)
     kd = t.from_numpy(KS).cuda()
-    sp_b = _spectra_dev(t, cfg, 8, 501)
+    sp_b = _spectra_dev(t, cfg, 8, 501, slope=slope)
     plan = SphTransPlan(cfg, max_ncomp=9, precision=SHT_PREC_FP64)
     try:
         grid_f = t.empty(cfg.n_rtp(9), dtype=t.float64, device="cuda")
-        plan.sph_backward_transforms(3, 0, _spectra_dev(t, cfg, 9, 503), grid_f)
+        plan.sph_backward_transforms(3, 0, _spectra_dev(t, cfg, 9, 503, slope=slope), grid_f)
         t.cuda.synchronize()
     finally:
         plan.close()
