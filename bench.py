@@ -263,7 +263,18 @@ def run_engine(cfg, precision, args, torch, sp_b, grid_f, keep=False, otf=False,
                         p_on_the_fly=otf)
     plan.set_stream(stream.cuda_stream)
     apply_variants(plan, args)
-    tuned = plan.autotune(2, 2, 3, 0, cache=args.autotune) if args.autotune else None
+    tuned = None
+    if args.autotune:
+        from kemorin_mhd_amd import ShtError
+        from kemorin_mhd_amd.sht import SHT_ERR_OOM
+        try:
+            tuned = plan.autotune(2, 2, 3, 0, cache=args.autotune)
+        except ShtError as e:
+            # the timing runs need field buffers of their own; where they do
+            # not fit (config 4 at full size) the plan keeps its default forms
+            if e.status != SHT_ERR_OOM:
+                raise
+            tuned = {"skipped": str(e)}
     dev = "cuda"
     with torch.cuda.stream(stream):
         # aliased: the backward's 8 grid components are the forward's first 8

@@ -160,6 +160,7 @@ extern "C" int sht_plan_autotune(sht_plan *p, int nvb, int nsb, int nvf, int nsf
   if (hipMalloc(&sp, sizeof(double) * nc * nsp) != hipSuccess ||
       hipMalloc(&grid, sizeof(double) * nc * nnod) != hipSuccess) {
     fail(SHT_ERR_OOM, "device buffers for the timing runs");
+    (void)hipGetLastError();  // the failed allocation must not fail the next launch check
   } else if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
     fail(SHT_ERR_HIP, "events");
   } else {

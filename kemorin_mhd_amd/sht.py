@@ -148,6 +148,7 @@ def _check(rc: int, where: str):
 HIP_STREAM_LEGACY = 1  # hipStreamLegacy: the legacy default (null) stream
 SHT_FLAG_P_ON_THE_FLY = 1  # P tables generated per call, one tile of orders at a time
 SHT_FLAG_CHECK_FINITE = 2  # whole transforms count their outputs' NaN / Inf values
+SHT_ERR_OOM = 3            # status of a call whose device allocation failed
 SHT_ERR_NONFINITE = 6      # status of a checked transform whose output is not finite
 
 
