@@ -174,7 +174,11 @@ def _ptr(t) -> int:
 def chebyshev_radius(nr: int, ri: float, ro: float) -> np.ndarray:
     """Chebyshev-Gauss-Lobatto radii of the reference's shell grid
     (MHD/Fortran_src/CORE_src/SPH_SPECTR_src/chebyshev_radial_grid.f90,
-    set_chebyshev_distance_shell)."""
+    set_chebyshev_distance_shell).  The reference's spacing divides by
+    nr - 1; a one-radius grid (a single spherical surface) is taken here as
+    the inner radius."""
+    if nr == 1:
+        return np.array([ri], dtype=np.float64)
     k = np.arange(nr, dtype=np.float64)
     return ri + 0.5 * (ro - ri) * (1.0 - np.cos(np.pi * k / (nr - 1)))
 

@@ -51,12 +51,33 @@ def _rel_comp(a, b, ncomp, comp_fastest):
 @pytest.mark.parametrize("cfg", SHELLS, ids=[f"L{c.L}_nr{c.nr}_nth{c.nth}_nphi{c.nphi}"
                                              for c in SHELLS])
 def test_whole_transform_sweep(torch_cuda, cfg):
-    t = torch_cuda
     _say(f"[sweep] L={cfg.L} nr={cfg.nr} nth={cfg.nth} nphi={cfg.nphi}")
-    orc = O.OraclePlan(cfg.L, cfg.nr, cfg.nth, cfg.nphi, cfg.radius(), full_tables=False)
     from kemorin_mhd_amd import synthetic_spectra
-    sp = synthetic_spectra(cfg, 2, 2, seed=cfg.L + 17)
-    spf = synthetic_spectra(cfg, 3, 0, seed=cfg.L + 29)
+    _check(torch_cuda, cfg, synthetic_spectra(cfg, 2, 2, seed=cfg.L + 17),
+           synthetic_spectra(cfg, 3, 0, seed=cfg.L + 29))
+
+
+# The smallest shells the plan accepts: L from 1, one to three radii, the
+# minimum Ntheta = max(2, L+1) (and one above it, odd) and Nphi = 2L+2 or a
+# little more, whose factors 3, 5 and 7 go to the generic FFT's prime passes.
+# One- and two-radius shells are all boundary, where the synthetic spectra's
+# radial envelope vanishes, so these take plain seeded Gaussian spectra.
+TINY = [ShellConfig(1, 1, 2, 4), ShellConfig(2, 1, 3, 6), ShellConfig(3, 2, 4, 8),
+        ShellConfig(4, 1, 5, 10), ShellConfig(5, 3, 9, 14), ShellConfig(1, 2, 3, 6)]
+
+
+@pytest.mark.parametrize("cfg", TINY, ids=[f"L{c.L}_nr{c.nr}_nth{c.nth}_nphi{c.nphi}"
+                                           for c in TINY])
+def test_tiny_shells(torch_cuda, cfg):
+    _say(f"[tiny] L={cfg.L} nr={cfg.nr} nth={cfg.nth} nphi={cfg.nphi}")
+    rng = np.random.default_rng(1000 + 10 * cfg.L + cfg.nr)
+    _check(torch_cuda, cfg, rng.standard_normal(cfg.n_rj(8)), rng.standard_normal(cfg.n_rj(9)))
+
+
+def _check(t, cfg, sp, spf):
+    """Backward 2 vector + 2 scalar of sp, and forward 3 vector of the
+    oracle's backward of spf, on both engines against the oracle."""
+    orc = O.OraclePlan(cfg.L, cfg.nr, cfg.nth, cfg.nphi, cfg.radius(), full_tables=False)
     ref_v = orc.backward(2, 2, sp)
     grid = orc.backward(3, 0, spf)
     ref_s = orc.forward(3, 0, grid)
