@@ -6,7 +6,8 @@ the component-group paths of the kernels (vector groups, scalars fused into
 the vector launch of the split backward, scalar-only calls) run here at the
 full 64 components: 20 vectors + 4 scalars, 21 vectors + 1 scalar and 64
 scalars each way, on a shell whose FFT takes the generic form and one whose
-FFT takes the split form.  Inputs are seeded Gaussian spectra and grids.
+FFT takes the split form, with stored P tables and with P generated per
+call (SHT_FLAG_P_ON_THE_FLY).  Inputs are seeded Gaussian spectra and grids.
 Bars against the oracle, per component: FP64 relative L2 1e-12, split
 (6 slices) 1e-11."""
 import numpy as np
@@ -34,17 +35,18 @@ def _rel_comp(a, b, ncomp, comp_fastest):
 
 @pytest.mark.parametrize("prec,tol", [(SHT_PREC_FP64, 1e-12), (SHT_PREC_SPLIT_INT8, 1e-11)],
                          ids=["fp64", "split"])
+@pytest.mark.parametrize("otf", [False, True], ids=["tables", "p_on_the_fly"])
 @pytest.mark.parametrize("name", list(SHELLS))
-def test_64_components_vs_oracle(torch_cuda, name, prec, tol):
+def test_64_components_vs_oracle(torch_cuda, name, otf, prec, tol):
     t = torch_cuda
     cfg = SHELLS[name]
     orc = O.OraclePlan(cfg.L, cfg.nr, cfg.nth, cfg.nphi, cfg.radius(), full_tables=False)
     rng = np.random.default_rng(640 + cfg.L)
-    plan = SphTransPlan(cfg, max_ncomp=64, precision=prec)
+    plan = SphTransPlan(cfg, max_ncomp=64, precision=prec, p_on_the_fly=otf)
     try:
         for nv, ns in FIELD_SETS:
             nc = 3 * nv + ns
-            print(f"[gpu] {name} precision={prec} nv={nv} ns={ns}", flush=True)
+            print(f"[gpu] {name} otf={otf} precision={prec} nv={nv} ns={ns}", flush=True)
             sp = rng.standard_normal(cfg.n_rj(nc))
             grid = rng.standard_normal(cfg.n_rtp(nc))
             ref_b = orc.backward(nv, ns, sp)
